@@ -1,0 +1,234 @@
+/*
+ * nhf.h -- C ABI of libnhf_hip.so: the self-consistent Skyrme HF(+BCS/HFB)
+ * iteration on a 3D lattice, on one MI355X (gfx950).
+ *
+ * Data layout at this boundary (identical to the reference's Eigen objects):
+ *   - spatial index    p = i + n*(j + n*k)          (x fastest; grid.hpp:87-89)
+ *   - spinor row       r = s + 2*p                  (spin fastest; grid.hpp:90-92)
+ *   - a block of m states is column-major: state c starts at c*2*n^3 rows,
+ *     complex128 stored as interleaved (re, im) doubles (Eigen::MatrixXcd)
+ *   - vector fields (grad f, W, grad rho) are 3 planes of n^3 doubles
+ *     (Eigen::MatrixX3d, column-major); J is 9 planes, column mu*3+nu
+ *     (Real2Tensor, wavefunction.cpp:451)
+ *
+ * Ownership: input pointers are caller-owned and read only during the call;
+ * outputs are caller-allocated.  The context owns every device buffer.
+ * Threading: one host thread per context; all work is enqueued on the
+ * context's own HIP stream.  Errors are negative integers (NHF_E*).
+ */
+#ifndef NHF_H
+#define NHF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NHF_OK 0
+#define NHF_EINVAL (-1)   /* bad argument (odd mesh, null pointer, sizes) */
+#define NHF_EHIP (-2)     /* HIP runtime error */
+#define NHF_ENOMEM (-3)   /* device allocation failed */
+#define NHF_ESTATE (-4)   /* call out of order (e.g. no fields set) */
+#define NHF_ESOLVER (-5)  /* dense eigensolver / Cholesky failure */
+#define NHF_ENODEV (-6)   /* no HIP device */
+
+/* precision of the H.psi stencil / block products */
+#define NHF_PREC_FP64 0   /* plain path: FP64 everywhere */
+#define NHF_PREC_MIXED 1  /* FP32 stencil + CG, split-BF16 MFMA Gram/projection,
+                             FP64 Cholesky-QR, Rayleigh-Ritz, lambda, energies */
+
+#define NHF_PAIR_NONE 0
+#define NHF_PAIR_BCS 1
+#define NHF_PAIR_HFB 2
+
+/* Skyrme EDF coupling constants, in the order of FunctionalParameters
+ * (reference include/EDF.hpp:25-28). */
+#define NHF_EDF_C0RR 0
+#define NHF_EDF_C1RR 1
+#define NHF_EDF_C0DRR 2
+#define NHF_EDF_C1DRR 3
+#define NHF_EDF_C0RT 4
+#define NHF_EDF_C1RT 5
+#define NHF_EDF_C0J2 6
+#define NHF_EDF_C1J2 7
+#define NHF_EDF_C0RDR 8
+#define NHF_EDF_C1RDR 9
+#define NHF_EDF_C0NJ 10
+#define NHF_EDF_C1NJ 11
+#define NHF_EDF_SIGMA 12
+#define NHF_EDF_N 13
+
+/* Pairing parameters of one species (reference PairingParameters,
+ * include/input_parser.hpp:61-69; window default 5, windowBoth default true). */
+typedef struct nhf_pairing {
+  double V0;
+  double window;
+  int window_both;
+  int active;
+} nhf_pairing;
+
+/* Block-GCG parameters (reference GCGParameters, input_parser.hpp:31-37;
+ * HF values from parameters/minimization_parameters.json). */
+typedef struct nhf_gcg_params {
+  int max_iter;   /* GCG passes per call (HF: 1) */
+  int steps;      /* inner Jacobi-PCG steps (HF: 80) */
+  double tol;     /* residual-norm locking tolerance (HF: 1e-6) */
+  double cg_tol;  /* inner CG relative tolerance (HF: 1e-5) */
+} nhf_gcg_params;
+
+/* Kinds of multipole constraint (reference src/skyrme/ *_constraint.cpp). */
+#define NHF_CON_XCM 0
+#define NHF_CON_YCM 1
+#define NHF_CON_ZCM 2
+#define NHF_CON_X2MY2 3
+#define NHF_CON_XY2 4
+#define NHF_CON_OCT 5
+#define NHF_CON_Q20 6
+#define NHF_MAX_CON 8
+
+typedef struct nhf_constraint {
+  int kind;        /* NHF_CON_* */
+  double target;   /* target expectation value */
+  double C;        /* stiffness (reference defaults: COM 0.05, Q20 0.005,
+                      x2-y2/xy 0.005, octupole 0.001) */
+} nhf_constraint;
+
+/* One SCF problem (the subset of the reference InputParser used by the
+ * HF loop, input_parser.cpp:50-261, plus the two guess/iteration knobs the
+ * configs name). */
+typedef struct nhf_config {
+  int n;                 /* axisMeshPoints (even, >= 6) */
+  double box_size;       /* boxSize: full width, a = box_size / 2 */
+  int A, Z;
+  int M_n, M_p;          /* states per species (N + additional, Z + additional) */
+  double edf[NHF_EDF_N];
+  int spin_orbit;        /* W field from grad rho */
+  int use_j2;            /* W field from J (J^2 terms) */
+  int com_corr;          /* m -> m A / (A-1) */
+  int coulomb;
+  double density_mix;    /* default 0.25 */
+  int max_iter;          /* maxIterations (HF cycles) */
+  double energy_tol;     /* MeV */
+  nhf_gcg_params gcg;
+  int pairing_type;      /* NHF_PAIR_* */
+  nhf_pairing pair_n, pair_p;
+  int start_hfb_iter;    /* default 3 */
+  double pairing_threshold; /* default 1e-3 */
+  int n_con;
+  nhf_constraint con[NHF_MAX_CON];
+  double constraints_tol; /* default 1e-3 */
+  int precision;         /* NHF_PREC_* */
+  double stop_energy;    /* if finite: stop once |E_int - stop_energy| <= stop_window */
+  double stop_window;    /* MeV (1e-3 = 1 keV); ignored when stop_energy is NaN */
+} nhf_config;
+
+/* Terms of the total energy (reference iteration_data.cpp:153-271 and
+ * iteration_data_integrals.cpp:95-262). */
+#define NHF_E_C0RHO 0
+#define NHF_E_C1RHO 1
+#define NHF_E_C0LAP 2
+#define NHF_E_C1LAP 3
+#define NHF_E_C0TAU 4
+#define NHF_E_C1TAU 5
+#define NHF_E_C0J2 6
+#define NHF_E_C1J2 7
+#define NHF_E_C0DIVJ 8
+#define NHF_E_C1DIVJ 9
+#define NHF_E_COUL_DIR 10
+#define NHF_E_COUL_EX 11
+#define NHF_E_KIN 12
+#define NHF_E_PAIR_N 13
+#define NHF_E_PAIR_P 14
+#define NHF_E_REAR 15
+#define NHF_E_CONSTR 16
+#define NHF_E_INT 17      /* E_INT = totalEnergyIntegral() + kineticEnergy() */
+#define NHF_E_HF 18       /* HFEnergy() */
+#define NHF_NTERMS 19
+
+/* Result of an SCF run. */
+typedef struct nhf_result {
+  int iterations;        /* HF iterations executed */
+  int converged;         /* energy criterion met (main.cpp:333-337) */
+  double terms[NHF_NTERMS];
+  double r2_n, r2_p;     /* <r^2> of neutrons / protons (fm^2) */
+  double beta2, gamma;   /* quadrupole deformation (gamma in radians) */
+  double lambda_n, lambda_p;
+  double seconds;        /* wall time of the iteration loop */
+  double seconds_to_stop;/* wall time until the stop_energy window was met (or -1) */
+} nhf_result;
+
+/* ---------------------------------------------------------------- context */
+
+/* Create a context on HIP device `device`.  Replaces nothing in the
+ * reference (which has no device); owns a stream and all device memory. */
+int nhf_create(void **out_ctx, int device);
+void nhf_destroy(void *ctx);
+const char *nhf_strerror(int code);
+
+/* Register an SCF problem: lattice, sizes, functional, pairing.
+ * Replaces InputParser::get_grid + Grid(n,a) (input_parser.cpp:269-272,
+ * grid.cpp:7-21) and IterationData(InputParser) (iteration_data.cpp:80-87).
+ * Returns NHF_EINVAL for an odd or < 6 mesh, A < Z, odd A or Z. */
+int nhf_problem(void *ctx, const nhf_config *cfg, int *pid);
+
+/* Upload the single-particle fields of species q (0 = neutrons, 1 = protons):
+ * f = hbar^2/2m*, grad f (3 planes), U (central + Coulomb + constraint,
+ * already summed), W (3 planes: the spin-orbit form factor B_q).
+ * Replaces constructing SkyrmeU/NonLocalKinetic/LocalKinetic/SkyrmeSO/
+ * LocalCoulomb with a shared IterationData (utilities.cpp:56-70). */
+int nhf_set_fields(void *ctx, int pid, int q, const double *f, const double *gradf,
+                   const double *U, const double *W);
+
+/* hpsi = H_q psi for a block of m states (matrix-free 13-point stencil).
+ * Replaces Hamiltonian::buildMatrix() + the SpMV applications
+ * (hamiltonian.cpp:303-372, solver.cpp:431,741,766,796).  prec = NHF_PREC_*. */
+int nhf_hpsi(void *ctx, int pid, int q, const double *psi, double *hpsi, int m, int prec);
+
+/* Densities of species q from m states weighted by occ[c] (v^2 of column c):
+ * rho, tau (n^3), J (9 planes), div J (n^3).  Not mixed.
+ * Replaces Wavefunction::density/kineticDensity/soDensity/divJ
+ * (wavefunction.cpp:21-44,348-360,421-464,466-517). */
+int nhf_densities(void *ctx, int pid, const double *psi, int m, const double *occ,
+                  double *rho, double *tau, double *J9, double *divJ);
+
+/* Coulomb direct potential by CG on the 4th-order Laplacian with the
+ * reference's multipole ghost-shell boundary (wavefunction.cpp:55-229).
+ * U is the warm start on input (zeros for none) and the solution on output. */
+int nhf_poisson(void *ctx, int pid, const double *rho_p, double *U, double rel_tol,
+                int max_iter, int *iters_out);
+
+/* One block-GCG call for species q with the fields last set by
+ * nhf_set_fields: mirrors gcgm_complex_no_B_lock (solver.cpp:642-959) with
+ * ASCENDING_ENERGIES ordering, as called by Utilities::solve
+ * (utilities.cpp:30-36).  psi_inout: 2n^3 x m guess in / eigenvectors out;
+ * conjdir: 2n^3 x p_cols (may be NULL when p_cols == 0); eps_out: m values. */
+int nhf_gcg_step(void *ctx, int pid, int q, const double *conjdir, int p_cols,
+                 const nhf_gcg_params *params, double *psi_inout, int m,
+                 double *eps_out, int prec);
+
+/* Full self-consistent loop (main.cpp:199-342) on the device.
+ * psi_n / psi_p: initial states (2n^3 x M_n / M_p), overwritten with the
+ * final states; eps_n / eps_p: final single-particle energies.
+ * trace (optional, may be NULL): per iteration [E_int, E_HF, max|d eps|,
+ * seconds since start], trace_cap rows. */
+int nhf_scf_run(void *ctx, int pid, double *psi_n, double *psi_p, double *eps_n,
+                double *eps_p, nhf_result *res, double *trace, int trace_cap);
+
+/* Measurement: average duration (ms, HIP events on the context stream) of
+ * `reps` back-to-back H.psi stencil launches over m device-resident states
+ * of species q in precision prec (must equal the problem precision).
+ * No reference counterpart (the reference has no timers on this path). */
+int nhf_bench_hpsi(void *ctx, int pid, int q, int m, int prec, int reps, double *avg_ms);
+
+/* Harmonic-oscillator spinor guess |n l j m_j> with oscillator length b
+ * (fm), Euclidean-normalised columns (guess.cpp:10-248 with
+ * useSpinOrbit = true).  Host-side; out: 2n^3 x nev complex. */
+int nhf_ho_guess(int n, double box_size, int nev, double b, double *out);
+
+/* Skyrme (t,x,W0,sigma) -> EDF coupling constants (EDF.cpp:76-98).
+ * skyrme: t0 t1 t2 t3 x0 x1 x2 x3 W0 W0' te to sigma. */
+void nhf_edf_from_skyrme(const double skyrme[13], double edf[NHF_EDF_N]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NHF_H */

@@ -1,0 +1,256 @@
+"""Host-side configuration: ctypes mirrors of the structs in include/nhf.h,
+the JSON input contract of the reference's InputParser
+(src/system/input_parser.cpp:50-261), and the named workloads.
+
+The JSON keys and defaults follow the reference parser exactly:
+functional, outputName, outputDirectory, maxIterations, energyTol,
+box.{boxSize, axisMeshPoints}, nucleus.{A, Z}, coulombInteraction (true),
+densityMix (0.25), startHFBIter (3), constrainCOM (false), constraintsTol
+(1e-3), pairingThreshold (1e-3), pairing{type, neutron/proton{HFbasisSize,
+V0, window=5, windowBoth=true} | flat form}, J2/spinOrbit/COMCorrection
+(functional file overrides input, default true).
+"""
+import ctypes
+import json
+import math
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FUNCTIONAL_DIR = os.path.join(HERE, "functionals")
+
+HBAR = 197.327053  # MeV fm (constants.hpp:12)
+MNUC = 938.91869   # MeV (constants.hpp:14)
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+PAIR_NONE, PAIR_BCS, PAIR_HFB = 0, 1, 2
+CON_XCM, CON_YCM, CON_ZCM, CON_X2MY2, CON_XY2, CON_OCT, CON_Q20 = range(7)
+MAX_CON = 8
+NTERMS = 19
+TERM_NAMES = ["C0rho", "C1rho", "C0lap", "C1lap", "C0tau", "C1tau", "C0J2", "C1J2", "C0divJ", "C1divJ",
+              "coul_dir", "coul_ex", "kin", "pair_n", "pair_p", "rear", "constr", "E_int", "E_HF"]
+E_INT, E_HF = 17, 18
+
+
+class NhfPairing(ctypes.Structure):
+    _fields_ = [("V0", ctypes.c_double), ("window", ctypes.c_double), ("window_both", ctypes.c_int),
+                ("active", ctypes.c_int)]
+
+
+class NhfGcgParams(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int), ("steps", ctypes.c_int), ("tol", ctypes.c_double),
+                ("cg_tol", ctypes.c_double)]
+
+
+class NhfConstraint(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("target", ctypes.c_double), ("C", ctypes.c_double)]
+
+
+class NhfConfig(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("box_size", ctypes.c_double), ("A", ctypes.c_int), ("Z", ctypes.c_int),
+        ("M_n", ctypes.c_int), ("M_p", ctypes.c_int), ("edf", ctypes.c_double * 13),
+        ("spin_orbit", ctypes.c_int), ("use_j2", ctypes.c_int), ("com_corr", ctypes.c_int), ("coulomb", ctypes.c_int),
+        ("density_mix", ctypes.c_double), ("max_iter", ctypes.c_int), ("energy_tol", ctypes.c_double),
+        ("gcg", NhfGcgParams), ("pairing_type", ctypes.c_int), ("pair_n", NhfPairing), ("pair_p", NhfPairing),
+        ("start_hfb_iter", ctypes.c_int), ("pairing_threshold", ctypes.c_double), ("n_con", ctypes.c_int),
+        ("con", NhfConstraint * MAX_CON), ("constraints_tol", ctypes.c_double), ("precision", ctypes.c_int),
+        ("stop_energy", ctypes.c_double), ("stop_window", ctypes.c_double),
+    ]
+
+    @property
+    def a(self):
+        return 0.5 * self.box_size
+
+    @property
+    def h(self):
+        return self.box_size / (self.n - 1)
+
+
+class NhfResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("terms", ctypes.c_double * NTERMS),
+                ("r2_n", ctypes.c_double), ("r2_p", ctypes.c_double), ("beta2", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("lambda_n", ctypes.c_double), ("lambda_p", ctypes.c_double),
+                ("seconds", ctypes.c_double), ("seconds_to_stop", ctypes.c_double)]
+
+    def summary(self):
+        return {
+            "iterations": self.iterations, "converged": bool(self.converged),
+            "E_int": self.terms[E_INT], "E_HF": self.terms[E_HF],
+            "terms": {k: self.terms[i] for i, k in enumerate(TERM_NAMES)},
+            "rms_n": math.sqrt(max(self.r2_n, 0.0)), "rms_p": math.sqrt(max(self.r2_p, 0.0)),
+            "beta2": self.beta2, "gamma": self.gamma, "lambda_n": self.lambda_n, "lambda_p": self.lambda_p,
+            "seconds": self.seconds, "seconds_to_stop": self.seconds_to_stop,
+        }
+
+
+def edf_from_skyrme(par):
+    """Skyrme (t, x, W0, sigma) -> C coefficients (EDF.cpp:59-98), in the
+    FunctionalParameters order of include/nhf.h."""
+    t0, t1, t2, t3 = par["t0"], par["t1"], par["t2"], par["t3"]
+    x0, x1, x2, x3 = par["x0"], par["x1"], par["x2"], par["x3"]
+    W0 = par["W0"]
+    W0p = par.get("W0_pr", W0)
+    if "te" in par:
+        te, to = par["te"], par["to"]
+    elif "aT" in par:
+        to = 4 * par["aT"] / 5
+        te = 8 * par["bT"] / 5 - to
+    else:
+        te = to = 0.0
+    C0T = -(t1 * (1 - 2 * x1) - t2 * (1 + 2 * x2)) / 16
+    C1T = -(t1 - t2) / 16
+    C0F = -3 * (te + 3 * to) / 8
+    C1F = -3 * (te - to) / 8
+    return [
+        3 * t0 / 8, -t0 * (0.5 + x0) / 4, 3 * t3 / 48, -t3 * (0.5 + x3) / 24,
+        3 * t1 / 16 + t2 * (5.0 / 4.0 + x2) / 4, -t1 * (0.5 + x1) / 8 + t2 * (0.5 + x2) / 8,
+        -0.5 * (C0T - 0.5 * C0F), -0.5 * (C1T - 0.5 * C1F),
+        -9 * t1 / 64 + t2 * (5.0 / 4.0 + x2) / 16, 3 * t1 * (0.5 + x1) / 32 + t2 * (0.5 + x2) / 32,
+        -0.5 * W0 - 0.25 * W0p, -0.5 * W0, par["sigma"],
+    ]
+
+
+def load_functional(name):
+    with open(os.path.join(FUNCTIONAL_DIR, name + ".json")) as fh:
+        par = json.load(fh)
+    if "t0" in par:
+        return edf_from_skyrme(par), par
+    keys = ["C0rr", "C1rr", "C0Drr", "C1Drr", "C0rt", "C1rt", "C0J2", "C1J2", "C0rdr", "C1rdr", "C0nJ", "C1nJ",
+            "sigma"]
+    return [par[k] for k in keys], par
+
+
+def hf_gcg_params():
+    """parameters/minimization_parameters.json of the reference."""
+    return NhfGcgParams(max_iter=1, steps=80, tol=1e-6, cg_tol=1e-5)
+
+
+def config_from_input(data, precision=PREC_FP64):
+    """InputParser (input_parser.cpp:50-261) -> NhfConfig.  `data` is the
+    parsed JSON dict of one input file."""
+    cfg = NhfConfig()
+    edf, fpar = load_functional(data["functional"])
+    A, Z = int(data["nucleus"]["A"]), int(data["nucleus"]["Z"])
+    N = A - Z
+    cfg.n = int(data["box"]["axisMeshPoints"])
+    cfg.box_size = float(data["box"]["boxSize"])
+    cfg.A, cfg.Z = A, Z
+    for i, v in enumerate(edf):
+        cfg.edf[i] = v
+
+    def flag(key):
+        if key in fpar:
+            return bool(fpar[key])
+        if key in data:
+            return bool(data[key])
+        return True
+
+    cfg.use_j2 = int(flag("J2"))
+    cfg.spin_orbit = int(flag("spinOrbit"))
+    cfg.com_corr = int(flag("COMCorrection"))
+    cfg.coulomb = int(data.get("coulombInteraction", True))
+    cfg.density_mix = float(data.get("densityMix", 0.25))
+    cfg.max_iter = int(data["maxIterations"])
+    cfg.energy_tol = float(data["energyTol"])
+    cfg.gcg = hf_gcg_params()
+    cfg.start_hfb_iter = int(data.get("startHFBIter", 3))
+    cfg.pairing_threshold = float(data.get("pairingThreshold", 1e-3))
+    cfg.constraints_tol = float(data.get("constraintsTol", 1e-3))
+    cfg.precision = precision
+    cfg.stop_energy = float("nan")
+    cfg.stop_window = 1e-3
+    add_n = add_p = 0
+    cfg.pairing_type = PAIR_NONE
+    if "pairing" in data:
+        pd = data["pairing"]
+        cfg.pairing_type = {"HFB": PAIR_HFB, "BCS": PAIR_BCS}.get(pd.get("type"), PAIR_NONE)
+
+        def species(d, particles):
+            p = NhfPairing(V0=float(d["V0"]), window=float(d.get("window", 5.0)),
+                           window_both=int(d.get("windowBoth", True)), active=1)
+            return p, int(d["HFbasisSize"]) - particles
+
+        inactive = NhfPairing(0.0, 0.0, 0, 0)
+        if "neutron" in pd or "proton" in pd:
+            cfg.pair_n, add_n = species(pd["neutron"], N) if "neutron" in pd else (inactive, 0)
+            cfg.pair_p, add_p = species(pd["proton"], Z) if "proton" in pd else (inactive, 0)
+        else:
+            bn = pd.get("HFbasisSizeN", pd.get("HFbasisSize"))
+            bp = pd.get("HFbasisSizeP", pd.get("HFbasisSize"))
+            common = dict(V0=pd["V0"], window=pd["window"], windowBoth=pd.get("windowBoth", True))
+            cfg.pair_n, add_n = species(dict(common, HFbasisSize=bn), N)
+            cfg.pair_p, add_p = species(dict(common, HFbasisSize=bp), Z)
+        if cfg.pairing_type == PAIR_NONE:
+            cfg.pair_n = NhfPairing(0.0, 0.0, 0, 0)
+            cfg.pair_p = NhfPairing(0.0, 0.0, 0, 0)
+    cfg.M_n = N + add_n
+    cfg.M_p = Z + add_p
+    cons = []
+    if data.get("constrainCOM", False):
+        cons += [(CON_XCM, 0.0, 0.05), (CON_YCM, 0.0, 0.05), (CON_ZCM, 0.0, 0.05)]
+    cfg.n_con = len(cons)
+    for i, (k, t, c) in enumerate(cons):
+        cfg.con[i] = NhfConstraint(k, t, c)
+    return cfg
+
+
+def check(cfg):
+    """InputParser::check (input_parser.cpp:7-48)."""
+    if cfg.A % 2 or cfg.Z % 2:
+        return False
+    if cfg.A < cfg.Z:
+        return False
+    if cfg.n % 2:
+        return False
+    if cfg.pairing_type != PAIR_NONE:
+        for p, M, Nq in ((cfg.pair_n, cfg.M_n, cfg.A - cfg.Z), (cfg.pair_p, cfg.M_p, cfg.Z)):
+            if p.active and (p.V0 < 0 or M - Nq <= 0 or (M - Nq) % 2):
+                return False
+    return True
+
+
+def ho_length(A):
+    """Oscillator length b for hbar*omega = 41 A^(-1/3) MeV."""
+    hw = 41.0 * A ** (-1.0 / 3.0)
+    return HBAR / math.sqrt(MNUC * hw)
+
+
+def ho_energies(nev, A):
+    """hbar*omega (N + 3/2) of the |n l j m_j> orbitals in the order of the
+    guess (guess.cpp:129-163): used as the starting single-particle energies."""
+    hw = 41.0 * A ** (-1.0 / 3.0)
+    out = []
+    N = 0
+    while len(out) < nev:
+        for nr in range(N // 2 + 1):
+            l = N - 2 * nr
+            cnt = (2 * l + 2) + (2 * l if l > 0 else 0)
+            out += [hw * (N + 1.5)] * cnt
+        N += 1
+    return out[:nev]
+
+
+def workload(name, precision=PREC_FP64, seed=None):
+    """The named synthetic workloads (nucleus, functional, lattice, states).
+
+    o16:  O-16 spherical HF, SLy4, box [-9,9] fm, 30 pts/axis, 12+12 states
+    ca40: Ca-40 HF+BCS, SLy4, box [-9,9] fm, 40 pts/axis, 32+32 states,
+          V0 = 200 + 100 U(0,1) MeV fm^3 from the seed
+    """
+    import random
+
+    if name == "o16":
+        d = {"functional": "SLy4", "maxIterations": 600, "energyTol": 1e-6,
+             "box": {"boxSize": 18.0, "axisMeshPoints": 30}, "nucleus": {"A": 16, "Z": 8}}
+        cfg = config_from_input(d, precision)
+        cfg.M_n = cfg.M_p = 12
+        return cfg
+    if name == "ca40":
+        rng = random.Random(1234 if seed is None else seed)
+        V0 = 200.0 + 100.0 * rng.random()
+        d = {"functional": "SLy4", "maxIterations": 600, "energyTol": 1e-6,
+             "box": {"boxSize": 18.0, "axisMeshPoints": 40}, "nucleus": {"A": 40, "Z": 20},
+             "pairing": {"type": "BCS", "window": 5.0, "V0": V0, "HFbasisSize": 32}}
+        return config_from_input(d, precision)
+    raise KeyError(name)

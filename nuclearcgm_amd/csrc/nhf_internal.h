@@ -1,0 +1,143 @@
+// nhf_internal.h -- declarations shared by the HIP kernels, the dense host
+// linear algebra and the SCF driver of libnhf_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/nhf.h"
+
+namespace nhf {
+
+using cd = std::complex<double>;
+
+// One lattice point of a two-component spinor: (up re, up im, down re, down im).
+template <typename R>
+struct alignas(4 * sizeof(R)) Sp {
+  R ur, ui, dr, di;
+};
+
+// Number of field scalars per lattice point packed for the stencil:
+// f, df/dx, df/dy, df/dz, U, Wx, Wy, Wz.
+constexpr int NFLD = 8;
+
+// Scalars accumulated per CG iteration and column:
+// [0..1] p^H A p (re, im), [2] |r|^2, [3] r^H D^-1 r.
+constexpr int CG_NS = 4;
+
+// ------------------------------------------------------------------ kernels
+// All launches go to `st`.  Sizes: N3 = n^3 lattice points, a column of
+// spinors has N3 Sp<R> entries; `ld` is the column stride in Sp<R> units.
+template <typename R>
+void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int ncols, const R* fld, int n, double h,
+                 double shift, double* dot, const int* done);
+
+// block Jacobi-PCG kernels (Eigen ConjugateGradient restated per column)
+template <typename R>
+void launch_cg_init(hipStream_t st, const Sp<R>* X, const double* lam_shift, Sp<R>* W, Sp<R>* Rr, Sp<R>* P,
+                    const Sp<R>* T, const R* invd, size_t N3, size_t ld, int ncols, double* s0, double* rhs2);
+template <typename R>
+void launch_cg_update(hipStream_t st, Sp<R>* W, Sp<R>* Rr, const Sp<R>* P, const Sp<R>* T, const R* invd, size_t N3,
+                      size_t ld, int ncols, const double* sprev, double* scur, const int* done);
+template <typename R>
+void launch_cg_dir(hipStream_t st, const Sp<R>* Rr, Sp<R>* P, const R* invd, size_t N3, size_t ld, int ncols,
+                   const double* sprev, const double* scur, const double* thr, int* done, int* its);
+void launch_cg_check0(hipStream_t st, const double* s0, const double* rhs2, double tol, int ncols, double* thr,
+                      int* done);
+
+// C (ka x kb, complex double, column-major, leading dim ka) = A^H B where
+// A, B are columns of Sp<R> (rows = 2 N3 complex).  part: scratch of
+// nchunk * ka * kb complex doubles.
+template <typename R>
+void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>* B, size_t ldb, int kb, size_t N3,
+                 double2* part, int nchunk, double2* C);
+int gram_chunks(size_t N3);
+
+// Y (ncols) = V (k cols) * C (k x ncols complex double, column-major),
+// Y += ... when accumulate; when AV != null also accumulates
+// |AV c - w_c V c|^2 into rn2[c] (zeroed by the caller).
+template <typename R>
+void launch_rotate(hipStream_t st, const Sp<R>* V, size_t ldv, int k, const double2* C, int ncols, Sp<R>* Y,
+                   size_t ldy, size_t N3, const Sp<R>* AV, const double* w, double* rn2, int accumulate);
+
+template <typename R>
+void launch_axpby_cols(hipStream_t st, const Sp<R>* X, Sp<R>* Y, size_t N3, size_t ld, int ncols, const double* a,
+                       const double* b);  // Y_c = a_c X_c + b_c Y_c (real a, b per column)
+template <typename R>
+void launch_col_norms(hipStream_t st, const Sp<R>* X, size_t N3, size_t ld, int ncols, int n, int trapezoid,
+                      double* out);
+template <typename R>
+void launch_col_scale(hipStream_t st, Sp<R>* X, size_t N3, size_t ld, int ncols, const double* s);
+template <typename Ra, typename Rb>
+void launch_convert(hipStream_t st, const Sp<Ra>* a, Sp<Rb>* b, size_t count);
+template <typename R>
+void launch_time_reverse(hipStream_t st, const Sp<R>* X, Sp<R>* Y, size_t count);
+template <typename R>
+void launch_invdiag(hipStream_t st, const R* fld, size_t N3, double h, double shift, R* invd);
+
+// densities of one species: rho, tau, J(9), divJ accumulated with weights
+// occ[c] (device), then mixed into the stored values: X <- (1-mu) X + mu X_new
+// (mu = 1 when `first`).
+template <typename R>
+void launch_densities(hipStream_t st, const Sp<R>* psi, size_t ld, int m, const double* occ, int n, double h,
+                      double mu, int first, double* rho, double* tau, double* J9, double* divJ);
+
+struct FieldParams {
+  double edf[NHF_EDF_N];
+  double hb2m;  // hbar^2 / (2 m_corr)
+  int spin_orbit, use_j2, coulomb;
+  int ncon;
+  int con_kind[NHF_MAX_CON];
+  double con_coef[NHF_MAX_CON];
+};
+// derived fields of both species from the mixed densities: writes the
+// packed stencil fields fld64[q] (and fld32[q] when non-null), grad rho,
+// lap rho, J vector (3 planes) and the Coulomb potential U_C.
+void launch_fields(hipStream_t st, const FieldParams& P, int n, double a, double h, const double* const rho[2],
+                   const double* const tau[2], const double* const J9[2], const double* const divJ[2],
+                   const double* UCdir, double* grho[2], double* lrho[2], double* Jv[2], double* UCoul,
+                   double* fld64[2], float* fld32[2]);
+
+// weighted reductions: out[k] = int O_k(r) rho_tot(r) d^3r for the
+// constraint operators kind[k] (plus Q20/Q22 of the proton density for the
+// Coulomb boundary when q20 != null).
+void launch_moments(hipStream_t st, int n, double a, double h, const double* rho_n, const double* rho_p, int ncon,
+                    const int* kinds, double* out /* ncon */, double* q20q22 /* 2, may be null */);
+
+// Poisson (Coulomb): rhs, 13-point Laplacian CG pieces (real, FP64)
+void launch_coulomb_rhs(hipStream_t st, int n, double a, double h, int Z, const double* rho_p, const double* q20q22,
+                        double* rhs);
+void launch_lap4_dot(hipStream_t st, const double* u, double* out, int n, double h, double* dot);
+void launch_poisson_update(hipStream_t st, double* U, double* r, const double* p, const double* Ap, size_t N3,
+                           const double* scal, int slot, double* rr);
+void launch_poisson_dir(hipStream_t st, const double* r, double* p, size_t N3, const double* scal, int slot);
+void launch_poisson_init(hipStream_t st, const double* rhs, const double* AU, double* r, double* p, size_t N3,
+                         double* bn_rr);
+
+// energy integrands -> NHF energy term partial sums (trapezoid, FP64)
+void launch_energy(hipStream_t st, const FieldParams& P, int n, double h, const double* const rho[2],
+                   const double* const tau[2], const double* const lrho[2], const double* const Jv[2],
+                   const double* const divJ[2], const double* UCdir, double* out /* 16 */);
+// per-state densities of selected columns (for the BCS pairing matrix)
+template <typename R>
+void launch_state_density(hipStream_t st, const Sp<R>* psi, size_t ld, const int* cols, int ncols, size_t N3,
+                          double* out);
+void launch_pair_matrix(hipStream_t st, const double* dens, int P, int n, double h, double* G /* P*P */);
+void launch_r2(hipStream_t st, int n, double a, double h, const double* rho_n, const double* rho_p,
+               double* out /* 6: int r2 rho_n, int rho_n, int r2 rho_p, int rho_p, a20, a22 */);
+
+// ------------------------------------------------------- dense host algebra
+// Hermitian eigensolver reading the lower triangle of A (k x k, column
+// major), ascending eigenvalues.  Householder tridiagonalisation + implicit
+// QL.  Returns false on failure.
+bool eigh_lower(int k, const cd* A, double* w, cd* V);
+// Cholesky-QR coefficients with column dropping: given a Gram matrix G
+// (k x k Hermitian, lower triangle used) return the kept column list and
+// Cinv (k x ke) with V_eff = V * Cinv orthonormal.  drop_rel: a column is
+// dropped when its new component has squared norm <= drop_rel * G_jj.
+int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, std::vector<cd>& Cinv);
+
+}  // namespace nhf

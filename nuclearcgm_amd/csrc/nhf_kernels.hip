@@ -1,0 +1,1162 @@
+// nhf_kernels.hip -- hand-written gfx950 kernels of the self-consistent
+// Skyrme HF iteration.  Every kernel runs on 256-thread workgroups (four
+// 64-wide wavefronts); reductions go wavefront shuffle -> LDS -> one FP64
+// atomic per workgroup.
+//
+// Layout: a spinor column is n^3 points of Sp<R> = (up re, up im, down re,
+// down im), point index p = i + n (j + n k) (x fastest), so a wavefront reads
+// 64 consecutive points = one coalesced 1 KiB (FP32) / 2 KiB (FP64) segment.
+#include "nhf_internal.h"
+
+#include <cfloat>
+
+namespace nhf {
+
+constexpr int BS = 256;
+
+// ------------------------------------------------------------- helpers
+template <typename R>
+__device__ __forceinline__ Sp<R> sp_zero() {
+  return Sp<R>{R(0), R(0), R(0), R(0)};
+}
+template <typename R>
+__device__ __forceinline__ Sp<double> to_d(const Sp<R>& a) {
+  return Sp<double>{(double)a.ur, (double)a.ui, (double)a.dr, (double)a.di};
+}
+
+// sum over the workgroup of NV doubles per thread, then one atomic each
+template <int NV>
+__device__ __forceinline__ void block_atomic_add(double (&v)[NV], double* dst) {
+  __shared__ double sh[BS / 64][NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[wid][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) s += sh[w][threadIdx.x];
+    atomicAdd(dst + threadIdx.x, s);
+  }
+}
+
+__device__ __forceinline__ void decompose(size_t p, int n, int& i, int& j, int& k) {
+  const int nn = n * n;
+  k = (int)(p / (size_t)nn);
+  const int rem = (int)(p - (size_t)k * nn);
+  j = rem / n;
+  i = rem - j * n;
+}
+
+// trapezoid weight (1/2 per boundary face), without h^3
+__device__ __forceinline__ double trap_w(int i, int j, int k, int n) {
+  double w = 1.0;
+  if (i == 0 || i == n - 1) w *= 0.5;
+  if (j == 0 || j == n - 1) w *= 0.5;
+  if (k == 0 || k == n - 1) w *= 0.5;
+  return w;
+}
+
+// ================================================================ H psi
+// (H psi)_s = -f lap4 psi_s - grad f . D4 psi_s + (U + shift) psi_s
+//             - i sum_a sum_s' (sigma x W)_a,ss' D4_a psi_s'
+// lap4: 5-point 4th order per axis; D4: (8(+1 - -1) - (+2 - -2)) / 12h; all
+// truncated at the box (no ghost points); coefficients at the row point.
+template <typename R, bool DOT>
+__global__ __launch_bounds__(BS) void k_hpsi(const Sp<R>* __restrict__ in, Sp<R>* __restrict__ out, size_t ld,
+                                             const R* __restrict__ fld, int n, size_t N3, R c1, R c2, R shift,
+                                             double* __restrict__ dot, const int* __restrict__ done) {
+  const int c = blockIdx.y;
+  if (done && done[c]) return;  // uniform over the workgroup
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  const Sp<R>* __restrict__ src = in + (size_t)c * ld;
+  double acc[2] = {0.0, 0.0};
+  if (p < N3) {
+    int i, j, k;
+    decompose(p, n, i, j, k);
+    const Sp<R> a0 = src[p];
+    Sp<R> s16 = sp_zero<R>(), s1 = sp_zero<R>();  // sum of +-1 and +-2 neighbours
+    Sp<R> d[3];
+    const int pos[3] = {i, j, k};
+    const size_t stride[3] = {1, (size_t)n, (size_t)n * n};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const int q = pos[ax];
+      const size_t s = stride[ax];
+      const Sp<R> m2 = q >= 2 ? src[p - 2 * s] : sp_zero<R>();
+      const Sp<R> m1 = q >= 1 ? src[p - s] : sp_zero<R>();
+      const Sp<R> p1 = q + 1 < n ? src[p + s] : sp_zero<R>();
+      const Sp<R> p2 = q + 2 < n ? src[p + 2 * s] : sp_zero<R>();
+      s16.ur += p1.ur + m1.ur;
+      s16.ui += p1.ui + m1.ui;
+      s16.dr += p1.dr + m1.dr;
+      s16.di += p1.di + m1.di;
+      s1.ur += p2.ur + m2.ur;
+      s1.ui += p2.ui + m2.ui;
+      s1.dr += p2.dr + m2.dr;
+      s1.di += p2.di + m2.di;
+      d[ax].ur = (R(8) * (p1.ur - m1.ur) - (p2.ur - m2.ur)) * c1;
+      d[ax].ui = (R(8) * (p1.ui - m1.ui) - (p2.ui - m2.ui)) * c1;
+      d[ax].dr = (R(8) * (p1.dr - m1.dr) - (p2.dr - m2.dr)) * c1;
+      d[ax].di = (R(8) * (p1.di - m1.di) - (p2.di - m2.di)) * c1;
+    }
+    Sp<R> lap;
+    lap.ur = (R(16) * s16.ur - s1.ur - R(90) * a0.ur) * c2;
+    lap.ui = (R(16) * s16.ui - s1.ui - R(90) * a0.ui) * c2;
+    lap.dr = (R(16) * s16.dr - s1.dr - R(90) * a0.dr) * c2;
+    lap.di = (R(16) * s16.di - s1.di - R(90) * a0.di) * c2;
+    const R* __restrict__ F = fld + (size_t)NFLD * p;
+    const R f = F[0], gx = F[1], gy = F[2], gz = F[3], U = F[4] + shift, Wx = F[5], Wy = F[6], Wz = F[7];
+    // T_0 = -Wy dx0 - i Wz dx1 + Wx dy0 - Wz dy1 + (Wy + i Wx) dz1
+    // T_1 = i Wz dx0 + Wy dx1 - Wz dy0 - Wx dy1 + (Wy - i Wx) dz0
+    const R t0r = -Wy * d[0].ur + Wz * d[0].di + Wx * d[1].ur - Wz * d[1].dr + Wy * d[2].dr - Wx * d[2].di;
+    const R t0i = -Wy * d[0].ui - Wz * d[0].dr + Wx * d[1].ui - Wz * d[1].di + Wy * d[2].di + Wx * d[2].dr;
+    const R t1r = -Wz * d[0].ui + Wy * d[0].dr - Wz * d[1].ur - Wx * d[1].dr + Wy * d[2].ur + Wx * d[2].ui;
+    const R t1i = Wz * d[0].ur + Wy * d[0].di - Wz * d[1].ui - Wx * d[1].di + Wy * d[2].ui - Wx * d[2].ur;
+    Sp<R> o;
+    // -i T = (T.im, -T.re)
+    o.ur = -f * lap.ur - (gx * d[0].ur + gy * d[1].ur + gz * d[2].ur) + U * a0.ur + t0i;
+    o.ui = -f * lap.ui - (gx * d[0].ui + gy * d[1].ui + gz * d[2].ui) + U * a0.ui - t0r;
+    o.dr = -f * lap.dr - (gx * d[0].dr + gy * d[1].dr + gz * d[2].dr) + U * a0.dr + t1i;
+    o.di = -f * lap.di - (gx * d[0].di + gy * d[1].di + gz * d[2].di) + U * a0.di - t1r;
+    out[(size_t)c * ld + p] = o;
+    if (DOT) {
+      // conj(a0) . o
+      acc[0] = (double)a0.ur * o.ur + (double)a0.ui * o.ui + (double)a0.dr * o.dr + (double)a0.di * o.di;
+      acc[1] = (double)a0.ur * o.ui - (double)a0.ui * o.ur + (double)a0.dr * o.di - (double)a0.di * o.dr;
+    }
+  }
+  if (DOT) block_atomic_add<2>(acc, dot + (size_t)2 * c);
+}
+
+template <typename R>
+void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int ncols, const R* fld, int n, double h,
+                 double shift, double* dot, const int* done) {
+  size_t N3 = (size_t)n * n * n;
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  R c1 = (R)(1.0 / (12.0 * h)), c2 = (R)(1.0 / (12.0 * h * h));
+  if (dot)
+    hipLaunchKernelGGL((k_hpsi<R, true>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot, done);
+  else
+    hipLaunchKernelGGL((k_hpsi<R, false>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot,
+                       done);
+}
+template void launch_hpsi<double>(hipStream_t, const Sp<double>*, Sp<double>*, size_t, int, const double*, int, double,
+                                  double, double*, const int*);
+template void launch_hpsi<float>(hipStream_t, const Sp<float>*, Sp<float>*, size_t, int, const float*, int, double,
+                                 double, double*, const int*);
+
+// ======================================================== block Jacobi-PCG
+// Per column c the iteration of Eigen's ConjugateGradient with the diagonal
+// preconditioner (solver.cpp:733-743): scalars of iteration `it` live in
+// slot it+1 of scal[slot][c][CG_NS]; slot 0 holds the start values.
+
+template <typename R>
+__global__ __launch_bounds__(BS) void k_cg_init(const Sp<R>* __restrict__ X, const double* __restrict__ lam_shift,
+                                                Sp<R>* __restrict__ W, Sp<R>* __restrict__ Rr, Sp<R>* __restrict__ P,
+                                                const Sp<R>* __restrict__ T, const R* __restrict__ invd, size_t N3,
+                                                size_t ld, double* __restrict__ s0, double* __restrict__ rhs2) {
+  const int c = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};  // |b|^2, |r|^2, r^H z
+  if (p < N3) {
+    const size_t o = (size_t)c * ld + p;
+    const Sp<R> x = X[o], t = T[o];
+    const R ls = (R)lam_shift[c];
+    W[o] = x;
+    Sp<R> b{x.ur * ls, x.ui * ls, x.dr * ls, x.di * ls};
+    Sp<R> r{b.ur - t.ur, b.ui - t.ui, b.dr - t.dr, b.di - t.di};
+    Rr[o] = r;
+    const R dg = invd[p];
+    P[o] = Sp<R>{r.ur * dg, r.ui * dg, r.dr * dg, r.di * dg};
+    double bb = (double)b.ur * b.ur + (double)b.ui * b.ui + (double)b.dr * b.dr + (double)b.di * b.di;
+    double rr = (double)r.ur * r.ur + (double)r.ui * r.ui + (double)r.dr * r.dr + (double)r.di * r.di;
+    acc[0] = bb;
+    acc[1] = rr;
+    acc[2] = rr * (double)dg;
+  }
+  // reduce the three values into rhs2[c], s0[c][2], s0[c][3]
+  double v1[1] = {acc[0]};
+  block_atomic_add<1>(v1, rhs2 + c);
+  __syncthreads();
+  double v2[2] = {acc[1], acc[2]};
+  block_atomic_add<2>(v2, s0 + (size_t)c * CG_NS + 2);
+}
+
+template <typename R>
+void launch_cg_init(hipStream_t st, const Sp<R>* X, const double* lam_shift, Sp<R>* W, Sp<R>* Rr, Sp<R>* P,
+                    const Sp<R>* T, const R* invd, size_t N3, size_t ld, int ncols, double* s0, double* rhs2) {
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  hipLaunchKernelGGL((k_cg_init<R>), grid, dim3(BS), 0, st, X, lam_shift, W, Rr, P, T, invd, N3, ld, s0, rhs2);
+}
+
+__global__ void k_cg_check0(const double* __restrict__ s0, const double* __restrict__ rhs2, double tol, int ncols,
+                            double* __restrict__ thr, int* __restrict__ done) {
+  int c = threadIdx.x + blockIdx.x * blockDim.x;
+  if (c >= ncols) return;
+  double t = tol * tol * rhs2[c];
+  if (t < DBL_MIN) t = DBL_MIN;
+  thr[c] = t;
+  done[c] = (rhs2[c] == 0.0 || s0[(size_t)c * CG_NS + 2] < t) ? 1 : 0;
+}
+void launch_cg_check0(hipStream_t st, const double* s0, const double* rhs2, double tol, int ncols, double* thr,
+                      int* done) {
+  hipLaunchKernelGGL(k_cg_check0, dim3((ncols + 63) / 64), dim3(64), 0, st, s0, rhs2, tol, ncols, thr, done);
+}
+
+// x += alpha p; r -= alpha A p; alpha = (r^H z)_{prev} / (p^H A p)_{cur}
+template <typename R>
+__global__ __launch_bounds__(BS) void k_cg_update(Sp<R>* __restrict__ W, Sp<R>* __restrict__ Rr,
+                                                  const Sp<R>* __restrict__ P, const Sp<R>* __restrict__ T,
+                                                  const R* __restrict__ invd, size_t N3, size_t ld,
+                                                  const double* __restrict__ sprev, double* __restrict__ scur,
+                                                  const int* __restrict__ done) {
+  const int c = blockIdx.y;
+  if (done[c]) return;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  const double an = sprev[(size_t)c * CG_NS + 3];
+  const double pr = scur[(size_t)c * CG_NS + 0], pi = scur[(size_t)c * CG_NS + 1];
+  const double den = pr * pr + pi * pi;
+  const R ar = (R)(an * pr / den), ai = (R)(-an * pi / den);
+  double acc[2] = {0.0, 0.0};
+  if (p < N3) {
+    const size_t o = (size_t)c * ld + p;
+    const Sp<R> pp = P[o], t = T[o];
+    Sp<R> x = W[o], r = Rr[o];
+    x.ur += ar * pp.ur - ai * pp.ui;
+    x.ui += ar * pp.ui + ai * pp.ur;
+    x.dr += ar * pp.dr - ai * pp.di;
+    x.di += ar * pp.di + ai * pp.dr;
+    r.ur -= ar * t.ur - ai * t.ui;
+    r.ui -= ar * t.ui + ai * t.ur;
+    r.dr -= ar * t.dr - ai * t.di;
+    r.di -= ar * t.di + ai * t.dr;
+    W[o] = x;
+    Rr[o] = r;
+    double rr = (double)r.ur * r.ur + (double)r.ui * r.ui + (double)r.dr * r.dr + (double)r.di * r.di;
+    acc[0] = rr;
+    acc[1] = rr * (double)invd[p];
+  }
+  block_atomic_add<2>(acc, scur + (size_t)c * CG_NS + 2);
+}
+template <typename R>
+void launch_cg_update(hipStream_t st, Sp<R>* W, Sp<R>* Rr, const Sp<R>* P, const Sp<R>* T, const R* invd, size_t N3,
+                      size_t ld, int ncols, const double* sprev, double* scur, const int* done) {
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  hipLaunchKernelGGL((k_cg_update<R>), grid, dim3(BS), 0, st, W, Rr, P, T, invd, N3, ld, sprev, scur, done);
+}
+
+// convergence test |r|^2 < thr, else p = z + beta p
+template <typename R>
+__global__ __launch_bounds__(BS) void k_cg_dir(const Sp<R>* __restrict__ Rr, Sp<R>* __restrict__ P,
+                                               const R* __restrict__ invd, size_t N3, size_t ld,
+                                               const double* __restrict__ sprev, const double* __restrict__ scur,
+                                               const double* __restrict__ thr, int* __restrict__ done,
+                                               int* __restrict__ its) {
+  const int c = blockIdx.y;
+  if (done[c]) return;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (scur[(size_t)c * CG_NS + 2] < thr[c]) {
+    if (first) done[c] = 1;
+    return;
+  }
+  if (first) its[c] += 1;
+  const R beta = (R)(scur[(size_t)c * CG_NS + 3] / sprev[(size_t)c * CG_NS + 3]);
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p < N3) {
+    const size_t o = (size_t)c * ld + p;
+    const Sp<R> r = Rr[o];
+    Sp<R> q = P[o];
+    const R dg = invd[p];
+    q.ur = r.ur * dg + beta * q.ur;
+    q.ui = r.ui * dg + beta * q.ui;
+    q.dr = r.dr * dg + beta * q.dr;
+    q.di = r.di * dg + beta * q.di;
+    P[o] = q;
+  }
+}
+template <typename R>
+void launch_cg_dir(hipStream_t st, const Sp<R>* Rr, Sp<R>* P, const R* invd, size_t N3, size_t ld, int ncols,
+                   const double* sprev, const double* scur, const double* thr, int* done, int* its) {
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  hipLaunchKernelGGL((k_cg_dir<R>), grid, dim3(BS), 0, st, Rr, P, invd, N3, ld, sprev, scur, thr, done, its);
+}
+
+#define INST_CG(R)                                                                                                    \
+  template void launch_cg_init<R>(hipStream_t, const Sp<R>*, const double*, Sp<R>*, Sp<R>*, Sp<R>*, const Sp<R>*,    \
+                                  const R*, size_t, size_t, int, double*, double*);                                  \
+  template void launch_cg_update<R>(hipStream_t, Sp<R>*, Sp<R>*, const Sp<R>*, const Sp<R>*, const R*, size_t,       \
+                                    size_t, int, const double*, double*, const int*);                                \
+  template void launch_cg_dir<R>(hipStream_t, const Sp<R>*, Sp<R>*, const R*, size_t, size_t, int, const double*,    \
+                                 const double*, const double*, int*, int*);
+INST_CG(double)
+INST_CG(float)
+
+// diagonal of H + shift at each point (LocalKinetic diagonal + U): the
+// Jacobi preconditioner D^-1 (1 where the diagonal vanishes, as Eigen's
+// DiagonalPreconditioner)
+template <typename R>
+__global__ void k_invdiag(const R* __restrict__ fld, size_t N3, double c0, double shift, R* __restrict__ invd) {
+  size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  double dg = (double)fld[NFLD * p] * c0 + (double)fld[NFLD * p + 4] + shift;
+  invd[p] = (R)(dg != 0.0 ? 1.0 / dg : 1.0);
+}
+template <typename R>
+void launch_invdiag(hipStream_t st, const R* fld, size_t N3, double h, double shift, R* invd) {
+  hipLaunchKernelGGL((k_invdiag<R>), dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, fld, N3,
+                     (90.0 / 12.0) / (h * h), shift, invd);
+}
+template void launch_invdiag<double>(hipStream_t, const double*, size_t, double, double, double*);
+template void launch_invdiag<float>(hipStream_t, const float*, size_t, double, double, float*);
+
+// ================================================ tall-skinny block products
+// C = A^H B over 2 N3 complex rows.  Workgroup: a 16 x 16 output tile over
+// one chunk of GRAM_CHUNK points; rows are staged through LDS 32 points (64
+// complex rows) at a time; deterministic two-stage reduction over chunks.
+constexpr int GT = 16;
+constexpr int GRAM_CHUNK = 1024;
+constexpr int GROWS = 32;  // points per LDS stage
+
+int gram_chunks(size_t N3) { return (int)((N3 + GRAM_CHUNK - 1) / GRAM_CHUNK); }
+
+template <typename R>
+__global__ __launch_bounds__(BS) void k_gram(const Sp<R>* __restrict__ A, size_t lda, int ka,
+                                             const Sp<R>* __restrict__ B, size_t ldb, int kb, size_t N3,
+                                             double2* __restrict__ part) {
+  __shared__ double2 As[2 * GROWS][GT + 1];
+  __shared__ double2 Bs[2 * GROWS][GT + 1];
+  const int ntj = (kb + GT - 1) / GT;
+  const int ti0 = (blockIdx.x / ntj) * GT, tj0 = (blockIdx.x % ntj) * GT;
+  const int chunk = blockIdx.y;
+  const size_t p0 = (size_t)chunk * GRAM_CHUNK;
+  const size_t p1 = min(N3, p0 + GRAM_CHUNK);
+  const int tx = threadIdx.x % GT, ty = threadIdx.x / GT;
+  double sr = 0.0, si = 0.0;
+  for (size_t base = p0; base < p1; base += GROWS) {
+    // 2 * GROWS * GT = 1024 Sp loads per matrix? no: GROWS points x GT cols = 512 Sp
+    for (int e = threadIdx.x; e < GROWS * GT; e += BS) {
+      const int col = e / GROWS, pt = e % GROWS;
+      const size_t p = base + pt;
+      Sp<double> va = {0, 0, 0, 0}, vb = {0, 0, 0, 0};
+      if (p < p1) {
+        if (ti0 + col < ka) va = to_d(A[(size_t)(ti0 + col) * lda + p]);
+        if (tj0 + col < kb) vb = to_d(B[(size_t)(tj0 + col) * ldb + p]);
+      }
+      As[2 * pt][col] = make_double2(va.ur, va.ui);
+      As[2 * pt + 1][col] = make_double2(va.dr, va.di);
+      Bs[2 * pt][col] = make_double2(vb.ur, vb.ui);
+      Bs[2 * pt + 1][col] = make_double2(vb.dr, vb.di);
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int r = 0; r < 2 * GROWS; ++r) {
+      const double2 a = As[r][tx], b = Bs[r][ty];
+      // conj(a) * b
+      sr += a.x * b.x + a.y * b.y;
+      si += a.x * b.y - a.y * b.x;
+    }
+    __syncthreads();
+  }
+  const int i = ti0 + tx, j = tj0 + ty;
+  if (i < ka && j < kb) part[((size_t)chunk * kb + j) * ka + i] = make_double2(sr, si);
+}
+
+__global__ void k_reduce_parts(const double2* __restrict__ part, int nchunk, size_t cnt, double2* __restrict__ C) {
+  size_t e = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (e >= cnt) return;
+  double sr = 0.0, si = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    double2 v = part[(size_t)c * cnt + e];
+    sr += v.x;
+    si += v.y;
+  }
+  C[e] = make_double2(sr, si);
+}
+
+template <typename R>
+void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>* B, size_t ldb, int kb, size_t N3,
+                 double2* part, int nchunk, double2* C) {
+  int nti = (ka + GT - 1) / GT, ntj = (kb + GT - 1) / GT;
+  hipLaunchKernelGGL((k_gram<R>), dim3(nti * ntj, nchunk), dim3(BS), 0, st, A, lda, ka, B, ldb, kb, N3, part);
+  size_t cnt = (size_t)ka * kb;
+  hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((cnt + BS - 1) / BS)), dim3(BS), 0, st, part, nchunk, cnt, C);
+}
+template void launch_gram<double>(hipStream_t, const Sp<double>*, size_t, int, const Sp<double>*, size_t, int, size_t,
+                                  double2*, int, double2*);
+template void launch_gram<float>(hipStream_t, const Sp<float>*, size_t, int, const Sp<float>*, size_t, int, size_t,
+                                 double2*, int, double2*);
+
+// Y[:, c] (+)= sum_l V[:, l] C[l, c]; optional residual norms
+// |AV c - w_c V c|^2 (the Ritz residual of column c).
+constexpr int RT = 8;  // output columns per workgroup
+template <typename R, bool RES>
+__global__ __launch_bounds__(BS) void k_rotate(const Sp<R>* __restrict__ V, size_t ldv, int k,
+                                               const double2* __restrict__ C, int ncols, Sp<R>* __restrict__ Y,
+                                               size_t ldy, size_t N3, const Sp<R>* __restrict__ AV,
+                                               const double* __restrict__ w, double* __restrict__ rn2, int accumulate) {
+  extern __shared__ double2 Cs[];  // k x RT
+  const int c0 = blockIdx.y * RT;
+  const int nc = min(RT, ncols - c0);
+  for (int e = threadIdx.x; e < k * RT; e += BS) {
+    const int l = e / RT, cc = e % RT;
+    Cs[e] = (cc < nc) ? C[(size_t)(c0 + cc) * k + l] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double res[RT];
+#pragma unroll
+  for (int cc = 0; cc < RT; ++cc) res[cc] = 0.0;
+  if (p < N3) {
+    double y[RT][4], ay[RT][4];
+#pragma unroll
+    for (int cc = 0; cc < RT; ++cc)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        y[cc][q] = 0.0;
+        ay[cc][q] = 0.0;
+      }
+    for (int l = 0; l < k; ++l) {
+      const Sp<double> v = to_d(V[(size_t)l * ldv + p]);
+      Sp<double> av = {0, 0, 0, 0};
+      if (RES) av = to_d(AV[(size_t)l * ldv + p]);
+#pragma unroll
+      for (int cc = 0; cc < RT; ++cc) {
+        const double2 cf = Cs[l * RT + cc];
+        y[cc][0] += v.ur * cf.x - v.ui * cf.y;
+        y[cc][1] += v.ur * cf.y + v.ui * cf.x;
+        y[cc][2] += v.dr * cf.x - v.di * cf.y;
+        y[cc][3] += v.dr * cf.y + v.di * cf.x;
+        if (RES) {
+          ay[cc][0] += av.ur * cf.x - av.ui * cf.y;
+          ay[cc][1] += av.ur * cf.y + av.ui * cf.x;
+          ay[cc][2] += av.dr * cf.x - av.di * cf.y;
+          ay[cc][3] += av.dr * cf.y + av.di * cf.x;
+        }
+      }
+    }
+#pragma unroll
+    for (int cc = 0; cc < RT; ++cc) {
+      if (cc < nc) {
+        Sp<R>* dst = Y + (size_t)(c0 + cc) * ldy + p;
+        Sp<R> o;
+        if (accumulate) {
+          Sp<R> old = *dst;
+          o = Sp<R>{(R)(y[cc][0] + old.ur), (R)(y[cc][1] + old.ui), (R)(y[cc][2] + old.dr), (R)(y[cc][3] + old.di)};
+        } else {
+          o = Sp<R>{(R)y[cc][0], (R)y[cc][1], (R)y[cc][2], (R)y[cc][3]};
+        }
+        *dst = o;
+        if (RES) {
+          const double wc = w[c0 + cc];
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            double r = ay[cc][q] - wc * y[cc][q];
+            s += r * r;
+          }
+          res[cc] = s;
+        }
+      }
+    }
+  }
+  if (RES) block_atomic_add<RT>(res, rn2 + c0);
+}
+
+template <typename R>
+void launch_rotate(hipStream_t st, const Sp<R>* V, size_t ldv, int k, const double2* C, int ncols, Sp<R>* Y,
+                   size_t ldy, size_t N3, const Sp<R>* AV, const double* w, double* rn2, int accumulate) {
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)((ncols + RT - 1) / RT));
+  size_t sh = (size_t)k * RT * sizeof(double2);
+  if (AV)
+    hipLaunchKernelGGL((k_rotate<R, true>), grid, dim3(BS), sh, st, V, ldv, k, C, ncols, Y, ldy, N3, AV, w, rn2,
+                       accumulate);
+  else
+    hipLaunchKernelGGL((k_rotate<R, false>), grid, dim3(BS), sh, st, V, ldv, k, C, ncols, Y, ldy, N3, AV, w, rn2,
+                       accumulate);
+}
+template void launch_rotate<double>(hipStream_t, const Sp<double>*, size_t, int, const double2*, int, Sp<double>*,
+                                    size_t, size_t, const Sp<double>*, const double*, double*, int);
+template void launch_rotate<float>(hipStream_t, const Sp<float>*, size_t, int, const double2*, int, Sp<float>*, size_t,
+                                   size_t, const Sp<float>*, const double*, double*, int);
+
+// ------------------------------------------------------- column utilities
+template <typename R>
+__global__ void k_axpby(const Sp<R>* __restrict__ X, Sp<R>* __restrict__ Y, size_t N3, size_t ld, double a,
+                        double b) {
+  const int c = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  const size_t o = (size_t)c * ld + p;
+  Sp<R> x = X[o], y = Y[o];
+  R ar = (R)a, br = (R)b;
+  Y[o] = Sp<R>{ar * x.ur + br * y.ur, ar * x.ui + br * y.ui, ar * x.dr + br * y.dr, ar * x.di + br * y.di};
+}
+template <typename R>
+void launch_axpby_cols(hipStream_t st, const Sp<R>* X, Sp<R>* Y, size_t N3, size_t ld, int ncols, const double* a,
+                       const double* b) {
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  hipLaunchKernelGGL((k_axpby<R>), grid, dim3(BS), 0, st, X, Y, N3, ld, *a, *b);
+}
+template void launch_axpby_cols<double>(hipStream_t, const Sp<double>*, Sp<double>*, size_t, size_t, int,
+                                        const double*, const double*);
+template void launch_axpby_cols<float>(hipStream_t, const Sp<float>*, Sp<float>*, size_t, size_t, int, const double*,
+                                       const double*);
+
+template <typename R>
+__global__ __launch_bounds__(BS) void k_col_norms(const Sp<R>* __restrict__ X, size_t N3, size_t ld, int n, int trap,
+                                                  double* __restrict__ out) {
+  const int c = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double v[1] = {0.0};
+  if (p < N3) {
+    Sp<R> x = X[(size_t)c * ld + p];
+    double s = (double)x.ur * x.ur + (double)x.ui * x.ui + (double)x.dr * x.dr + (double)x.di * x.di;
+    if (trap) {
+      int i, j, k;
+      decompose(p, n, i, j, k);
+      s *= trap_w(i, j, k, n);
+    }
+    v[0] = s;
+  }
+  block_atomic_add<1>(v, out + c);
+}
+template <typename R>
+void launch_col_norms(hipStream_t st, const Sp<R>* X, size_t N3, size_t ld, int ncols, int n, int trapezoid,
+                      double* out) {
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  hipLaunchKernelGGL((k_col_norms<R>), grid, dim3(BS), 0, st, X, N3, ld, n, trapezoid, out);
+}
+template void launch_col_norms<double>(hipStream_t, const Sp<double>*, size_t, size_t, int, int, int, double*);
+template void launch_col_norms<float>(hipStream_t, const Sp<float>*, size_t, size_t, int, int, int, double*);
+
+template <typename R>
+__global__ void k_col_scale(Sp<R>* __restrict__ X, size_t N3, size_t ld, const double* __restrict__ s) {
+  const int c = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  const size_t o = (size_t)c * ld + p;
+  Sp<R> x = X[o];
+  R f = (R)s[c];
+  X[o] = Sp<R>{x.ur * f, x.ui * f, x.dr * f, x.di * f};
+}
+template <typename R>
+void launch_col_scale(hipStream_t st, Sp<R>* X, size_t N3, size_t ld, int ncols, const double* s) {
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  hipLaunchKernelGGL((k_col_scale<R>), grid, dim3(BS), 0, st, X, N3, ld, s);
+}
+template void launch_col_scale<double>(hipStream_t, Sp<double>*, size_t, size_t, int, const double*);
+template void launch_col_scale<float>(hipStream_t, Sp<float>*, size_t, size_t, int, const double*);
+
+template <typename Ra, typename Rb>
+__global__ void k_convert(const Sp<Ra>* __restrict__ a, Sp<Rb>* __restrict__ b, size_t cnt) {
+  size_t e = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (e >= cnt) return;
+  Sp<Ra> v = a[e];
+  b[e] = Sp<Rb>{(Rb)v.ur, (Rb)v.ui, (Rb)v.dr, (Rb)v.di};
+}
+template <typename Ra, typename Rb>
+void launch_convert(hipStream_t st, const Sp<Ra>* a, Sp<Rb>* b, size_t count) {
+  hipLaunchKernelGGL((k_convert<Ra, Rb>), dim3((unsigned)((count + BS - 1) / BS)), dim3(BS), 0, st, a, b, count);
+}
+template void launch_convert<double, float>(hipStream_t, const Sp<double>*, Sp<float>*, size_t);
+template void launch_convert<float, double>(hipStream_t, const Sp<float>*, Sp<double>*, size_t);
+template void launch_convert<double, double>(hipStream_t, const Sp<double>*, Sp<double>*, size_t);
+template void launch_convert<float, float>(hipStream_t, const Sp<float>*, Sp<float>*, size_t);
+
+// T psi = i sigma_y K psi: (up, down) -> (conj(down), -conj(up))
+template <typename R>
+__global__ void k_time_reverse(const Sp<R>* __restrict__ X, Sp<R>* __restrict__ Y, size_t cnt) {
+  size_t e = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (e >= cnt) return;
+  Sp<R> v = X[e];
+  Y[e] = Sp<R>{v.dr, -v.di, -v.ur, v.ui};
+}
+template <typename R>
+void launch_time_reverse(hipStream_t st, const Sp<R>* X, Sp<R>* Y, size_t count) {
+  hipLaunchKernelGGL((k_time_reverse<R>), dim3((unsigned)((count + BS - 1) / BS)), dim3(BS), 0, st, X, Y, count);
+}
+template void launch_time_reverse<double>(hipStream_t, const Sp<double>*, Sp<double>*, size_t);
+template void launch_time_reverse<float>(hipStream_t, const Sp<float>*, Sp<float>*, size_t);
+
+// ============================================================ densities
+// First derivative with the reference's boundary degradation
+// (differential_operators.cpp:356-417, 482-543): 2/3/5/7/9 points.
+template <typename T, typename G>
+__device__ __forceinline__ T d1(G get, int pos, int n, double h) {
+  if (pos == 0) return (get(1) - get(0)) * (1.0 / h);
+  if (pos == 1 || pos == n - 2) return (get(pos + 1) - get(pos - 1)) * (1.0 / (2.0 * h));
+  if (pos == n - 1) return (get(pos) - get(pos - 1)) * (1.0 / h);
+  if (pos == 2 || pos == n - 3)
+    return (get(pos - 2) - get(pos + 2) + 8.0 * (get(pos + 1) - get(pos - 1))) * (1.0 / (12.0 * h));
+  if (pos == 3 || pos == n - 4)
+    return ((get(pos + 3) - get(pos - 3)) - 9.0 * (get(pos + 2) - get(pos - 2)) + 45.0 * (get(pos + 1) - get(pos - 1))) *
+           (1.0 / (60.0 * h));
+  return (3.0 * (get(pos - 4) - get(pos + 4)) - 32.0 * (get(pos - 3) - get(pos + 3)) +
+          168.0 * (get(pos - 2) - get(pos + 2)) - 672.0 * (get(pos - 1) - get(pos + 1))) *
+         (1.0 / (840.0 * h));
+}
+// Second derivative (differential_operators.cpp:545-610) including the
+// first-difference expression at pos 0 and n-1.
+template <typename G>
+__device__ __forceinline__ double d2(G get, int pos, int n, double h) {
+  const double hh = h * h;
+  if (pos == 0) return (get(1) - get(0)) / h;
+  if (pos == 1 || pos == n - 2) return (get(pos - 1) - 2.0 * get(pos) + get(pos + 1)) / hh;
+  if (pos == n - 1) return (get(pos) - get(pos - 1)) / h;
+  if (pos == 2 || pos == n - 3)
+    return (-get(pos + 2) + 16.0 * get(pos + 1) - 30.0 * get(pos) + 16.0 * get(pos - 1) - get(pos - 2)) / (12.0 * hh);
+  if (pos == 3 || pos == n - 4)
+    return (2.0 * get(pos + 3) - 27.0 * get(pos + 2) + 270.0 * get(pos + 1) - 490.0 * get(pos) +
+            270.0 * get(pos - 1) - 27.0 * get(pos - 2) + 2.0 * get(pos - 3)) /
+           (180.0 * hh);
+  return (-9.0 * get(pos - 4) + 128.0 * get(pos - 3) - 1008.0 * get(pos - 2) + 8064.0 * get(pos - 1) -
+          14350.0 * get(pos) + 8064.0 * get(pos + 1) - 1008.0 * get(pos + 2) + 128.0 * get(pos + 3) -
+          9.0 * get(pos + 4)) /
+         (5040.0 * hh);
+}
+
+struct C4 {  // a spinor point in FP64 with arithmetic
+  double v[4];
+  __device__ C4 operator+(const C4& o) const { return C4{{v[0] + o.v[0], v[1] + o.v[1], v[2] + o.v[2], v[3] + o.v[3]}}; }
+  __device__ C4 operator-(const C4& o) const { return C4{{v[0] - o.v[0], v[1] - o.v[1], v[2] - o.v[2], v[3] - o.v[3]}}; }
+  __device__ C4 operator*(double s) const { return C4{{v[0] * s, v[1] * s, v[2] * s, v[3] * s}}; }
+};
+__device__ __forceinline__ C4 operator*(double s, const C4& a) { return a * s; }
+
+// a^dag sigma_l b  for spinors (re/im pairs)
+__device__ __forceinline__ double2 sig_prod(const C4& a, int l, const C4& b) {
+  // a0 = (a.v0, a.v1), a1 = (a.v2, a.v3)
+  double re, im;
+  if (l == 0) {  // conj(a0) b1 + conj(a1) b0
+    re = a.v[0] * b.v[2] + a.v[1] * b.v[3] + a.v[2] * b.v[0] + a.v[3] * b.v[1];
+    im = a.v[0] * b.v[3] - a.v[1] * b.v[2] + a.v[2] * b.v[1] - a.v[3] * b.v[0];
+  } else if (l == 1) {  // -i conj(a0) b1 + i conj(a1) b0
+    double xr = a.v[0] * b.v[2] + a.v[1] * b.v[3], xi = a.v[0] * b.v[3] - a.v[1] * b.v[2];  // conj(a0) b1
+    double yr = a.v[2] * b.v[0] + a.v[3] * b.v[1], yi = a.v[2] * b.v[1] - a.v[3] * b.v[0];  // conj(a1) b0
+    re = xi - yi;
+    im = -xr + yr;
+  } else {  // conj(a0) b0 - conj(a1) b1
+    re = a.v[0] * b.v[0] + a.v[1] * b.v[1] - a.v[2] * b.v[2] - a.v[3] * b.v[3];
+    im = a.v[0] * b.v[1] - a.v[1] * b.v[0] - a.v[2] * b.v[3] + a.v[3] * b.v[2];
+  }
+  return make_double2(re, im);
+}
+
+template <typename R>
+__global__ __launch_bounds__(BS) void k_densities(const Sp<R>* __restrict__ psi, size_t ld, int m,
+                                                  const double* __restrict__ occ, int n, double h, double mu,
+                                                  int first, double* __restrict__ rho, double* __restrict__ tau,
+                                                  double* __restrict__ J9, double* __restrict__ divJ) {
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  int i, j, k;
+  decompose(p, n, i, j, k);
+  const int pos[3] = {i, j, k};
+  const size_t stride[3] = {1, (size_t)n, (size_t)n * n};
+  double r = 0.0, t = 0.0, dj = 0.0, J[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) J[q] = 0.0;
+  for (int c = 0; c < m; ++c) {
+    const double w = occ[c];
+    if (w == 0.0) continue;
+    const Sp<R>* __restrict__ col = psi + (size_t)c * ld;
+    const Sp<double> a = to_d(col[p]);
+    const C4 a0{{a.ur, a.ui, a.dr, a.di}};
+    C4 g[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const size_t base = p - (size_t)pos[ax] * stride[ax];
+      const size_t s = stride[ax];
+      auto get = [&](int q) {
+        Sp<double> v = to_d(col[base + (size_t)q * s]);
+        return C4{{v.ur, v.ui, v.dr, v.di}};
+      };
+      g[ax] = d1<C4>(get, pos[ax], n, h);
+    }
+    r += w * (a0.v[0] * a0.v[0] + a0.v[1] * a0.v[1] + a0.v[2] * a0.v[2] + a0.v[3] * a0.v[3]);
+    double tt = 0.0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) tt += g[ax].v[q] * g[ax].v[q];
+    t += w * tt;
+#pragma unroll
+    for (int mu_ = 0; mu_ < 3; ++mu_)
+#pragma unroll
+      for (int nu = 0; nu < 3; ++nu) J[mu_ * 3 + nu] += w * sig_prod(a0, nu, g[mu_]).y;
+    // div J = -2 [Im(g_y^+ sz g_x) + Im(g_z^+ sx g_y) + Im(g_x^+ sy g_z)]
+    dj += -2.0 * w * (sig_prod(g[1], 2, g[0]).y + sig_prod(g[2], 0, g[1]).y + sig_prod(g[0], 1, g[2]).y);
+  }
+  const double a_ = first ? 1.0 : mu, b_ = first ? 0.0 : 1.0 - mu;
+  rho[p] = b_ * rho[p] + a_ * r;
+  tau[p] = b_ * tau[p] + a_ * t;
+  divJ[p] = b_ * divJ[p] + a_ * dj;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) J9[(size_t)q * N3 + p] = b_ * J9[(size_t)q * N3 + p] + a_ * J[q];
+}
+template <typename R>
+void launch_densities(hipStream_t st, const Sp<R>* psi, size_t ld, int m, const double* occ, int n, double h,
+                      double mu, int first, double* rho, double* tau, double* J9, double* divJ) {
+  size_t N3 = (size_t)n * n * n;
+  hipLaunchKernelGGL((k_densities<R>), dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, psi, ld, m, occ, n, h,
+                     mu, first, rho, tau, J9, divJ);
+}
+template void launch_densities<double>(hipStream_t, const Sp<double>*, size_t, int, const double*, int, double, double,
+                                       int, double*, double*, double*, double*);
+template void launch_densities<float>(hipStream_t, const Sp<float>*, size_t, int, const double*, int, double, double,
+                                      int, double*, double*, double*, double*);
+
+// =============================================================== fields
+constexpr double E2 = 1.43996446;
+constexpr double PI = 3.14159265358979323846;
+
+__device__ __forceinline__ double con_op(int kind, double x, double y, double z) {
+  switch (kind) {
+    case NHF_CON_XCM: return x;
+    case NHF_CON_YCM: return y;
+    case NHF_CON_ZCM: return z;
+    case NHF_CON_X2MY2: return x * x - y * y;
+    case NHF_CON_XY2: return 2.0 * x * y;
+    case NHF_CON_OCT: {
+      double r = sqrt(x * x + y * y + z * z), ct = z / r;
+      return r * r * r * sqrt(7.0 / (4 * PI)) * 0.5 * (5 * ct * ct * ct - 3 * ct);
+    }
+    default: return 0.25 * sqrt(5.0 / PI) * (2.0 * z * z - y * y - x * x);
+  }
+}
+
+struct FieldArgs {
+  FieldParams P;
+  const double* rho[2];
+  const double* tau[2];
+  const double* J9[2];
+  const double* divJ[2];
+  const double* UCdir;
+  double* grho[2];
+  double* lrho[2];
+  double* Jv[2];
+  double* UCoul;
+  double* fld64[2];
+  float* fld32[2];
+};
+
+__global__ __launch_bounds__(BS) void k_fields(FieldArgs A, int n, double a, double h) {
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  int i, j, k;
+  decompose(p, n, i, j, k);
+  const int pos[3] = {i, j, k};
+  const size_t stride[3] = {1, (size_t)n, (size_t)n * n};
+  const double* C = A.P.edf;
+  double gr[2][3], lr[2], jv[2][3];
+  for (int q = 0; q < 2; ++q) {
+    const double* rq = A.rho[q];
+    lr[q] = 0.0;
+    for (int ax = 0; ax < 3; ++ax) {
+      const size_t base = p - (size_t)pos[ax] * stride[ax];
+      const size_t s = stride[ax];
+      auto get = [&](int t) { return rq[base + (size_t)t * s]; };
+      double g = d1<double>(get, pos[ax], n, h);
+      gr[q][ax] = isnan(g) ? 0.0 : g;
+      lr[q] += d2(get, pos[ax], n, h);
+    }
+    const double* J = A.J9[q];
+    jv[q][0] = J[(1 * 3 + 2) * N3 + p] - J[(2 * 3 + 1) * N3 + p];
+    jv[q][1] = J[(2 * 3 + 0) * N3 + p] - J[(0 * 3 + 2) * N3 + p];
+    jv[q][2] = J[(0 * 3 + 1) * N3 + p] - J[(1 * 3 + 0) * N3 + p];
+    for (int ax = 0; ax < 3; ++ax) {
+      A.grho[q][ax * N3 + p] = gr[q][ax];
+      A.Jv[q][ax * N3 + p] = jv[q][ax];
+    }
+    A.lrho[q][p] = lr[q];
+  }
+  const double rho = A.rho[0][p] + A.rho[1][p];
+  const double tau = A.tau[0][p] + A.tau[1][p];
+  const double lap = lr[0] + lr[1];
+  const double dJn = A.P.spin_orbit ? A.divJ[0][p] : 0.0, dJp = A.P.spin_orbit ? A.divJ[1][p] : 0.0;
+  const double sig = C[NHF_EDF_SIGMA];
+  const double t3 = 48 * C[NHF_EDF_C0DRR] / 3;
+  const double x3 = -0.5 - 24 * C[NHF_EDF_C1DRR] / t3;
+  const double x = -a + i * h, y = -a + j * h, z = -a + k * h;
+  double ucon = 0.0;
+  for (int c = 0; c < A.P.ncon; ++c) ucon += A.P.con_coef[c] * con_op(A.P.con_kind[c], x, y, z);
+  double ucoul = 0.0;
+  if (A.P.coulomb) ucoul = A.UCdir[p] - E2 * cbrt(3.0 / PI) * cbrt(A.rho[1][p]);
+  A.UCoul[p] = ucoul;
+  const double rs = rho > 0.0 ? pow(rho, sig + 1.0) : 0.0;
+  for (int q = 0; q < 2; ++q) {
+    const double rq = A.rho[q][p], tq = A.tau[q][p], lq = lr[q], djq = q == 0 ? dJn : dJp;
+    // EffectiveMass (effective_mass.cpp:7-34)
+    const double f = A.P.hb2m + (C[NHF_EDF_C0RT] - C[NHF_EDF_C1RT]) * rho + 2 * C[NHF_EDF_C1RT] * rq;
+    double gf[3], W[3];
+    for (int ax = 0; ax < 3; ++ax) {
+      gf[ax] = (C[NHF_EDF_C0RT] - C[NHF_EDF_C1RT]) * (gr[0][ax] + gr[1][ax]) + 2 * C[NHF_EDF_C1RT] * gr[q][ax];
+      double w = 0.0;
+      if (A.P.spin_orbit)
+        w += -(C[NHF_EDF_C0NJ] - C[NHF_EDF_C1NJ]) * (gr[0][ax] + gr[1][ax]) - 2 * C[NHF_EDF_C1NJ] * gr[q][ax];
+      if (A.P.use_j2)
+        w += 2 * (C[NHF_EDF_C0J2] - C[NHF_EDF_C1J2]) * (jv[0][ax] + jv[1][ax]) + 4 * C[NHF_EDF_C1J2] * jv[q][ax];
+      W[ax] = w;
+    }
+    // Wavefunction::field (wavefunction.cpp:264-310); the density-dependent
+    // term rewritten as rho^(sigma+1) [ ... ] with t = rho_q / rho, which is
+    // algebraically identical and finite at rho = 0.
+    double U = (2 * C[NHF_EDF_C0RR] - 2 * C[NHF_EDF_C1RR]) * rho + 4 * C[NHF_EDF_C1RR] * rq;
+    U += (C[NHF_EDF_C0RT] - C[NHF_EDF_C1RT]) * tau + 2 * C[NHF_EDF_C1RT] * tq;
+    U += (2 * C[NHF_EDF_C0RDR] - 2 * C[NHF_EDF_C1RDR]) * lap + 4 * C[NHF_EDF_C1RDR] * lq;
+    U += (C[NHF_EDF_C0NJ] - C[NHF_EDF_C1NJ]) * (dJn + dJp) + 2 * C[NHF_EDF_C1NJ] * djq;
+    if (rho > 0.0) {
+      const double tt = rq / rho;
+      U += (t3 / 12.0) * rs *
+           ((1.0 + 0.5 * x3) * (sig + 2.0) - (x3 + 0.5) * (sig * (tt * tt + (1 - tt) * (1 - tt)) + 2.0 * tt));
+    }
+    U += ucon;
+    if (q == 1) U += ucoul;
+    double* F = A.fld64[q] + (size_t)NFLD * p;
+    F[0] = f;
+    F[1] = gf[0];
+    F[2] = gf[1];
+    F[3] = gf[2];
+    F[4] = U;
+    F[5] = W[0];
+    F[6] = W[1];
+    F[7] = W[2];
+    if (A.fld32[q]) {
+      float* G = A.fld32[q] + (size_t)NFLD * p;
+      for (int e = 0; e < NFLD; ++e) G[e] = (float)F[e];
+    }
+  }
+}
+
+void launch_fields(hipStream_t st, const FieldParams& P, int n, double a, double h, const double* const rho[2],
+                   const double* const tau[2], const double* const J9[2], const double* const divJ[2],
+                   const double* UCdir, double* grho[2], double* lrho[2], double* Jv[2], double* UCoul,
+                   double* fld64[2], float* fld32[2]) {
+  FieldArgs A;
+  A.P = P;
+  for (int q = 0; q < 2; ++q) {
+    A.rho[q] = rho[q];
+    A.tau[q] = tau[q];
+    A.J9[q] = J9[q];
+    A.divJ[q] = divJ[q];
+    A.grho[q] = grho[q];
+    A.lrho[q] = lrho[q];
+    A.Jv[q] = Jv[q];
+    A.fld64[q] = fld64[q];
+    A.fld32[q] = fld32 ? fld32[q] : nullptr;
+  }
+  A.UCdir = UCdir;
+  A.UCoul = UCoul;
+  size_t N3 = (size_t)n * n * n;
+  hipLaunchKernelGGL(k_fields, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, A, n, a, h);
+}
+
+// moments of the total density for the constraints and Q20/Q22 of the
+// proton density (coulombFieldPoisson, wavefunction.cpp:74-92)
+struct MomArgs {
+  int ncon;
+  int kind[NHF_MAX_CON];
+};
+__global__ __launch_bounds__(BS) void k_moments(MomArgs M, int n, double a, double h, const double* __restrict__ rn,
+                                                const double* __restrict__ rp, double* __restrict__ out,
+                                                double* __restrict__ q2) {
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double v[NHF_MAX_CON + 2];
+#pragma unroll
+  for (int e = 0; e < NHF_MAX_CON + 2; ++e) v[e] = 0.0;
+  if (p < N3) {
+    int i, j, k;
+    decompose(p, n, i, j, k);
+    const double w = trap_w(i, j, k, n) * h * h * h;
+    const double x = -a + i * h, y = -a + j * h, z = -a + k * h;
+    const double rho = rn[p] + rp[p];
+    for (int c = 0; c < M.ncon; ++c) v[c] = w * con_op(M.kind[c], x, y, z) * rho;
+    v[NHF_MAX_CON] = w * rp[p] * (2.0 * z * z - x * x - y * y) * sqrt(5.0 / (16.0 * PI));
+    v[NHF_MAX_CON + 1] = w * rp[p] * (x * x - y * y) * sqrt(15.0 / (32.0 * PI));
+  }
+  block_atomic_add<NHF_MAX_CON + 2>(v, out);
+  (void)q2;
+}
+void launch_moments(hipStream_t st, int n, double a, double h, const double* rho_n, const double* rho_p, int ncon,
+                    const int* kinds, double* out, double* q20q22) {
+  MomArgs M;
+  M.ncon = ncon;
+  for (int c = 0; c < NHF_MAX_CON; ++c) M.kind[c] = c < ncon ? kinds[c] : 0;
+  size_t N3 = (size_t)n * n * n;
+  // out must hold NHF_MAX_CON + 2 doubles (zeroed by the caller)
+  hipLaunchKernelGGL(k_moments, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, M, n, a, h, rho_n, rho_p, out,
+                     q20q22);
+}
+
+// ============================================================== Poisson
+// rhs = -4 pi e^2 rho_p minus the ghost-shell terms of the 13-point
+// Laplacian.  The boundary value of ghost node (X,Y,Z) is the multipole
+// potential evaluated at (c(Z), c(Y), c(X)), c(t) = -a + t h: the x <-> z
+// transposition of the reference (wavefunction.cpp:106-221).
+__device__ __forceinline__ double ubc(double x, double y, double z, int Z, double Q20, double Q22) {
+  double r2 = x * x + y * y + z * z;
+  double r = sqrt(r2);
+  if (r < 1e-9) r = 1e-9;
+  double ct = z / r;
+  double y20 = sqrt(5.0 / (16.0 * PI)) * (3.0 * ct * ct - 1.0);
+  double rho2 = x * x + y * y;
+  double c2p = rho2 > 0.0 ? (x * x - y * y) / rho2 : 1.0;
+  double y22 = sqrt(15.0 / (32.0 * PI)) * c2p * (1.0 - ct * ct);
+  return (E2 * Z) / r + E2 * (y20 * Q20 + 2.0 * y22 * Q22) / r;
+}
+__global__ __launch_bounds__(BS) void k_coulomb_rhs(int n, double a, double h, int Z, const double* __restrict__ rp,
+                                                    const double* __restrict__ mom, double* __restrict__ rhs) {
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  int i, j, k;
+  decompose(p, n, i, j, k);
+  const double Q20 = mom[NHF_MAX_CON], Q22 = mom[NHF_MAX_CON + 1];
+  const double hh = h * h;
+  double s = -(E2 * 4 * PI) * rp[p];
+  int G[3] = {i, j, k};
+  for (int ax = 0; ax < 3; ++ax)
+    for (int d = -2; d <= 2; ++d) {
+      if (d == 0) continue;
+      int q = G[ax] + d;
+      if (q >= 0 && q < n) continue;
+      int H[3] = {i, j, k};
+      H[ax] = q;
+      double cf = (d == 1 || d == -1) ? 16.0 / 12.0 : -1.0 / 12.0;
+      s -= cf * ubc(-a + H[2] * h, -a + H[1] * h, -a + H[0] * h, Z, Q20, Q22) / hh;
+    }
+  rhs[p] = s;
+}
+void launch_coulomb_rhs(hipStream_t st, int n, double a, double h, int Z, const double* rho_p, const double* mom,
+                        double* rhs) {
+  size_t N3 = (size_t)n * n * n;
+  hipLaunchKernelGGL(k_coulomb_rhs, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, n, a, h, Z, rho_p, mom,
+                     rhs);
+}
+
+// out = -lap4(u) (the SPD operator), dot += u . out
+__global__ __launch_bounds__(BS) void k_lap4(const double* __restrict__ u, double* __restrict__ out, int n, double h,
+                                             double* __restrict__ dot) {
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double v[1] = {0.0};
+  if (p < N3) {
+    int i, j, k;
+    decompose(p, n, i, j, k);
+    const int pos[3] = {i, j, k};
+    const size_t stride[3] = {1, (size_t)n, (size_t)n * n};
+    double s1 = 0.0, s2 = 0.0;
+    for (int ax = 0; ax < 3; ++ax) {
+      const int q = pos[ax];
+      const size_t s = stride[ax];
+      if (q >= 1) s1 += u[p - s];
+      if (q + 1 < n) s1 += u[p + s];
+      if (q >= 2) s2 += u[p - 2 * s];
+      if (q + 2 < n) s2 += u[p + 2 * s];
+    }
+    const double l = (16.0 * s1 - s2 - 90.0 * u[p]) / (12.0 * h * h);
+    out[p] = -l;
+    v[0] = -l * u[p];
+  }
+  if (dot) block_atomic_add<1>(v, dot);
+}
+void launch_lap4_dot(hipStream_t st, const double* u, double* out, int n, double h, double* dot) {
+  size_t N3 = (size_t)n * n * n;
+  hipLaunchKernelGGL(k_lap4, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, u, out, n, h, dot);
+}
+// scal[slot] = {pAp, rr}
+__global__ __launch_bounds__(BS) void k_poisson_update(double* __restrict__ U, double* __restrict__ r,
+                                                       const double* __restrict__ pv, const double* __restrict__ Ap,
+                                                       size_t N3, const double* __restrict__ scal, int slot,
+                                                       double* __restrict__ rr) {
+  const double pap = scal[2 * slot], rrp = scal[2 * (slot - 1) + 1];
+  const double al = pap != 0.0 ? rrp / pap : 0.0;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double v[1] = {0.0};
+  if (p < N3) {
+    U[p] += al * pv[p];
+    double rv = r[p] - al * Ap[p];
+    r[p] = rv;
+    v[0] = rv * rv;
+  }
+  block_atomic_add<1>(v, rr);
+}
+void launch_poisson_update(hipStream_t st, double* U, double* r, const double* p, const double* Ap, size_t N3,
+                           const double* scal, int slot, double* rr) {
+  hipLaunchKernelGGL(k_poisson_update, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, U, r, p, Ap, N3, scal,
+                     slot, rr);
+}
+__global__ void k_poisson_dir(const double* __restrict__ r, double* __restrict__ pv, size_t N3,
+                              const double* __restrict__ scal, int slot) {
+  const double rr = scal[2 * slot + 1], rrp = scal[2 * (slot - 1) + 1];
+  const double be = rrp != 0.0 ? rr / rrp : 0.0;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p < N3) pv[p] = r[p] + be * pv[p];
+}
+void launch_poisson_dir(hipStream_t st, const double* r, double* p, size_t N3, const double* scal, int slot) {
+  hipLaunchKernelGGL(k_poisson_dir, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, r, p, N3, scal, slot);
+}
+// r = b - A U with b = -rhs, A U = AU; p = r; bn_rr[0] += |b|^2, bn_rr[1] += |r|^2
+__global__ __launch_bounds__(BS) void k_poisson_init(const double* __restrict__ rhs, const double* __restrict__ AU,
+                                                     double* __restrict__ r, double* __restrict__ pv, size_t N3,
+                                                     double* __restrict__ bn_rr) {
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (p < N3) {
+    double b = -rhs[p];
+    double rv = b - AU[p];
+    r[p] = rv;
+    pv[p] = rv;
+    v[0] = b * b;
+    v[1] = rv * rv;
+  }
+  block_atomic_add<2>(v, bn_rr);
+}
+void launch_poisson_init(hipStream_t st, const double* rhs, const double* AU, double* r, double* p, size_t N3,
+                         double* bn_rr) {
+  hipLaunchKernelGGL(k_poisson_init, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, rhs, AU, r, p, N3, bn_rr);
+}
+
+// =============================================================== energies
+struct EnArgs {
+  FieldParams P;
+  const double* rho[2];
+  const double* tau[2];
+  const double* lrho[2];
+  const double* Jv[2];
+  const double* divJ[2];
+  const double* UCdir;
+};
+constexpr int NE = 16;
+__global__ __launch_bounds__(BS) void k_energy(EnArgs A, int n, double h, double* __restrict__ out) {
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double v[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) v[e] = 0.0;
+  if (p < N3) {
+    int i, j, k;
+    decompose(p, n, i, j, k);
+    const double w = trap_w(i, j, k, n) * h * h * h;
+    const double* C = A.P.edf;
+    const double sig = C[NHF_EDF_SIGMA];
+    const double rn = A.rho[0][p], rp = A.rho[1][p];
+    const double r0 = rn + rp, r1 = rn - rp;
+    const double ps = r0 > 0.0 ? pow(r0, sig) : 0.0;
+    v[NHF_E_C0RHO] = w * (C[NHF_EDF_C0RR] + C[NHF_EDF_C0DRR] * ps) * r0 * r0;
+    v[NHF_E_C1RHO] = w * (C[NHF_EDF_C1RR] + C[NHF_EDF_C1DRR] * ps) * r1 * r1;
+    v[NHF_E_C0LAP] = w * C[NHF_EDF_C0RDR] * r0 * (A.lrho[0][p] + A.lrho[1][p]);
+    v[NHF_E_C1LAP] = w * C[NHF_EDF_C1RDR] * (A.lrho[0][p] - A.lrho[1][p]) * r1;
+    const double tn = A.tau[0][p], tp = A.tau[1][p];
+    v[NHF_E_C0TAU] = w * C[NHF_EDF_C0RT] * (tn + tp) * r0;
+    v[NHF_E_C1TAU] = w * C[NHF_EDF_C1RT] * (tn - tp) * r1;
+    double j0 = 0.0, j1 = 0.0;
+    for (int ax = 0; ax < 3; ++ax) {
+      double a = A.Jv[0][ax * N3 + p], b = A.Jv[1][ax * N3 + p];
+      j0 += (a + b) * (a + b);
+      j1 += (a - b) * (a - b);
+    }
+    v[NHF_E_C0J2] = w * C[NHF_EDF_C0J2] * j0;
+    v[NHF_E_C1J2] = w * C[NHF_EDF_C1J2] * j1;
+    const double dn = A.divJ[0][p], dp = A.divJ[1][p];
+    v[NHF_E_C0DIVJ] = w * C[NHF_EDF_C0NJ] * (dn + dp) * r0;
+    v[NHF_E_C1DIVJ] = w * C[NHF_EDF_C1NJ] * (dn - dp) * r1;
+    if (A.P.coulomb) {
+      v[NHF_E_COUL_DIR] = w * 0.5 * rp * A.UCdir[p];
+      v[NHF_E_COUL_EX] = w * (-3.0 * E2 / 4.0) * cbrt(3.0 / PI) * (rp > 0.0 ? rp * cbrt(rp) : 0.0);
+    }
+    v[NHF_E_KIN] = w * A.P.hb2m * (tn + tp);
+    v[NHF_E_REAR] = w * (-0.5 * sig) * (C[NHF_EDF_C0DRR] * ps * r0 * r0 + C[NHF_EDF_C1DRR] * ps * r1 * r1);
+  }
+  block_atomic_add<NE>(v, out);
+}
+void launch_energy(hipStream_t st, const FieldParams& P, int n, double h, const double* const rho[2],
+                   const double* const tau[2], const double* const lrho[2], const double* const Jv[2],
+                   const double* const divJ[2], const double* UCdir, double* out) {
+  EnArgs A;
+  A.P = P;
+  for (int q = 0; q < 2; ++q) {
+    A.rho[q] = rho[q];
+    A.tau[q] = tau[q];
+    A.lrho[q] = lrho[q];
+    A.Jv[q] = Jv[q];
+    A.divJ[q] = divJ[q];
+  }
+  A.UCdir = UCdir;
+  size_t N3 = (size_t)n * n * n;
+  hipLaunchKernelGGL(k_energy, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, A, n, h, out);
+}
+
+// per-state densities |psi_c|^2 of the listed columns
+template <typename R>
+__global__ void k_state_density(const Sp<R>* __restrict__ psi, size_t ld, const int* __restrict__ cols, size_t N3,
+                                double* __restrict__ out) {
+  const int e = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  Sp<double> v = to_d(psi[(size_t)cols[e] * ld + p]);
+  out[(size_t)e * N3 + p] = v.ur * v.ur + v.ui * v.ui + v.dr * v.dr + v.di * v.di;
+}
+template <typename R>
+void launch_state_density(hipStream_t st, const Sp<R>* psi, size_t ld, const int* cols, int ncols, size_t N3,
+                          double* out) {
+  hipLaunchKernelGGL((k_state_density<R>), dim3((unsigned)((N3 + BS - 1) / BS), ncols), dim3(BS), 0, st, psi, ld, cols,
+                     N3, out);
+}
+template void launch_state_density<double>(hipStream_t, const Sp<double>*, size_t, const int*, int, size_t, double*);
+template void launch_state_density<float>(hipStream_t, const Sp<float>*, size_t, const int*, int, size_t, double*);
+
+// G[p][q] += int rho_p rho_q (trapezoid), one workgroup row-chunk per pair
+__global__ __launch_bounds__(BS) void k_pair_matrix(const double* __restrict__ dens, int P, int n, double h,
+                                                    double* __restrict__ G) {
+  const size_t N3 = (size_t)n * n * n;
+  const int pq = blockIdx.y;
+  const int pp = pq / P, qq = pq % P;
+  if (qq < pp) return;
+  double v[1] = {0.0};
+  for (size_t x = (size_t)blockIdx.x * BS + threadIdx.x; x < N3; x += (size_t)gridDim.x * BS) {
+    int i, j, k;
+    decompose(x, n, i, j, k);
+    v[0] += trap_w(i, j, k, n) * dens[(size_t)pp * N3 + x] * dens[(size_t)qq * N3 + x];
+  }
+  v[0] *= h * h * h;
+  block_atomic_add<1>(v, G + (size_t)qq * P + pp);
+}
+void launch_pair_matrix(hipStream_t st, const double* dens, int P, int n, double h, double* G) {
+  hipLaunchKernelGGL(k_pair_matrix, dim3(16, P * P), dim3(BS), 0, st, dens, P, n, h, G);
+}
+
+// observables: int r^2 rho_q, int rho_q, a20, a22
+__global__ __launch_bounds__(BS) void k_r2(int n, double a, double h, const double* __restrict__ rn,
+                                           const double* __restrict__ rp, double* __restrict__ out) {
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (p < N3) {
+    int i, j, k;
+    decompose(p, n, i, j, k);
+    const double w = trap_w(i, j, k, n) * h * h * h;
+    const double x = -a + i * h, y = -a + j * h, z = -a + k * h, r2 = x * x + y * y + z * z;
+    v[0] = w * r2 * rn[p];
+    v[1] = w * rn[p];
+    v[2] = w * r2 * rp[p];
+    v[3] = w * rp[p];
+    v[4] = w * sqrt(5.0 / (16.0 * PI)) * (2.0 * z * z - x * x - y * y) * (rn[p] + rp[p]);
+    v[5] = w * sqrt(15.0 / (32.0 * PI)) * (x * x - y * y) * (rn[p] + rp[p]);
+  }
+  block_atomic_add<6>(v, out);
+}
+void launch_r2(hipStream_t st, int n, double a, double h, const double* rho_n, const double* rho_p, double* out) {
+  size_t N3 = (size_t)n * n * n;
+  hipLaunchKernelGGL(k_r2, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, n, a, h, rho_n, rho_p, out);
+}
+
+}  // namespace nhf

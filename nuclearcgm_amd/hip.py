@@ -1,0 +1,187 @@
+"""ctypes binding of libnhf_hip.so (the C ABI of include/nhf.h).
+
+The HIP library is the only compute path: importing this module raises if
+the library has not been built (python -m nuclearcgm_amd.build), and every
+call raises NhfError on a negative return code.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+from .config import NhfConfig, NhfGcgParams, NhfResult, PREC_FP64
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnhf_hip.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+class NhfError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NhfError(f"{LIB_PATH} is missing: build it with `python -m nuclearcgm_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.nhf_strerror.restype = ctypes.c_char_p
+        L.nhf_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
+        L.nhf_destroy.argtypes = [ctypes.c_void_p]
+        L.nhf_problem.argtypes = [ctypes.c_void_p, ctypes.POINTER(NhfConfig), ctypes.POINTER(ctypes.c_int)]
+        L.nhf_set_fields.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]
+        L.nhf_hpsi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_int]
+        L.nhf_densities.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D, _D, _D, _D, _D]
+        L.nhf_poisson.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int)]
+        L.nhf_gcg_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int,
+                                   ctypes.POINTER(NhfGcgParams), _D, ctypes.c_int, _D, ctypes.c_int]
+        L.nhf_scf_run.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, ctypes.POINTER(NhfResult), _D,
+                                  ctypes.c_int]
+        L.nhf_bench_hpsi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, _D]
+        L.nhf_ho_guess.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]
+        L.nhf_edf_from_skyrme.argtypes = [_D, _D]
+        L.nhf_edf_from_skyrme.restype = None
+        _lib = L
+    return _lib
+
+
+def _chk(rc):
+    if rc != 0:
+        raise NhfError(f"nhf error {rc}: {lib().nhf_strerror(rc).decode()}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _cols(a):
+    return np.asfortranarray(a, dtype=np.complex128)
+
+
+def ho_guess(n, box_size, nev, b):
+    out = np.zeros((2 * n ** 3, nev), dtype=np.complex128, order="F")
+    _chk(lib().nhf_ho_guess(n, box_size, nev, b, _dp(out)))
+    return out
+
+
+def edf_from_skyrme(sk):
+    sk = np.ascontiguousarray(sk, dtype=np.float64)
+    out = np.zeros(13)
+    lib().nhf_edf_from_skyrme(_dp(sk), _dp(out))
+    return out
+
+
+def trapezoid_normalize(psi, n, box_size):
+    """Wavefunction::normalize (wavefunction.cpp:362-370) on the host."""
+    a = 0.5 * box_size
+    h = 2 * a / (n - 1)
+    w1 = np.ones(n)
+    w1[0] = w1[-1] = 0.5
+    w = (w1[None, None, :] * w1[None, :, None] * w1[:, None, None]).reshape(-1) * h ** 3  # x fastest
+    psi = np.asfortranarray(psi, dtype=np.complex128)
+    for c in range(psi.shape[1]):
+        d = np.abs(psi[0::2, c]) ** 2 + np.abs(psi[1::2, c]) ** 2
+        psi[:, c] /= math.sqrt(float(np.dot(w, d)))
+    return psi
+
+
+class Context:
+    """One HIP device, one stream (nhf_create / nhf_destroy)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _chk(lib().nhf_create(ctypes.byref(h), device))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().nhf_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def problem(self, cfg: NhfConfig):
+        pid = ctypes.c_int()
+        _chk(lib().nhf_problem(self.h, ctypes.byref(cfg), ctypes.byref(pid)))
+        return Problem(self, pid.value, cfg)
+
+
+class Problem:
+    def __init__(self, ctx, pid, cfg):
+        self.ctx, self.pid, self.cfg = ctx, pid, cfg
+        self.n = cfg.n
+        self.N3 = cfg.n ** 3
+
+    def set_fields(self, q, f, gradf, U, W):
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (f, gradf, U, W)]
+        _chk(lib().nhf_set_fields(self.ctx.h, self.pid, q, *[_dp(x) for x in arrs]))
+
+    def hpsi(self, q, psi, prec=PREC_FP64):
+        psi = _cols(psi)
+        if psi.ndim == 1:
+            psi = psi.reshape(-1, 1, order="F")
+        out = np.zeros_like(psi, order="F")
+        _chk(lib().nhf_hpsi(self.ctx.h, self.pid, q, _dp(psi), _dp(out), psi.shape[1], prec))
+        return out
+
+    def densities(self, psi, occ):
+        psi = _cols(psi)
+        m = psi.shape[1]
+        occ = np.ascontiguousarray(occ, dtype=np.float64)
+        rho, tau, divJ = (np.zeros(self.N3) for _ in range(3))
+        J = np.zeros(9 * self.N3)
+        _chk(lib().nhf_densities(self.ctx.h, self.pid, _dp(psi), m, _dp(occ), _dp(rho), _dp(tau), _dp(J),
+                                 _dp(divJ)))
+        return rho, tau, J, divJ
+
+    def poisson(self, rho_p, U0=None, tol=1e-13, maxit=20000):
+        U = np.zeros(self.N3) if U0 is None else np.array(U0, dtype=np.float64)
+        rho_p = np.ascontiguousarray(rho_p, dtype=np.float64)
+        it = ctypes.c_int()
+        _chk(lib().nhf_poisson(self.ctx.h, self.pid, _dp(rho_p), _dp(U), tol, maxit, ctypes.byref(it)))
+        return U, it.value
+
+    def gcg_step(self, q, psi, conj=None, params=None, prec=None):
+        psi = _cols(psi).copy(order="F")
+        m = psi.shape[1]
+        if params is None:
+            params = NhfGcgParams(1, 80, 1e-6, 1e-5)
+        pc = 0 if conj is None else conj.shape[1]
+        cj = _cols(conj) if conj is not None else np.zeros((1, 1), dtype=np.complex128, order="F")
+        eps = np.zeros(m)
+        _chk(lib().nhf_gcg_step(self.ctx.h, self.pid, q, _dp(cj), pc, ctypes.byref(params), _dp(psi), m, _dp(eps),
+                                self.cfg.precision if prec is None else prec))
+        return psi, eps
+
+    def bench_hpsi(self, q, m, prec=None, reps=50):
+        ms = ctypes.c_double()
+        _chk(lib().nhf_bench_hpsi(self.ctx.h, self.pid, q, m, self.cfg.precision if prec is None else prec, reps,
+                                  ctypes.byref(ms)))
+        return ms.value
+
+    def scf_run(self, psi_n, psi_p, eps_n, eps_p, trace_cap=4000):
+        psi_n = _cols(psi_n).copy(order="F")
+        psi_p = _cols(psi_p).copy(order="F")
+        eps_n = np.array(eps_n, dtype=np.float64)
+        eps_p = np.array(eps_p, dtype=np.float64)
+        res = NhfResult()
+        trace = np.zeros((trace_cap, 4))
+        _chk(lib().nhf_scf_run(self.ctx.h, self.pid, _dp(psi_n), _dp(psi_p), _dp(eps_n), _dp(eps_p),
+                               ctypes.byref(res), _dp(trace), trace_cap))
+        return psi_n, psi_p, eps_n, eps_p, res, trace[: min(res.iterations, trace_cap)].copy()

@@ -1,0 +1,1815 @@
+// nhf_oracle.cpp -- FP64 CPU oracle for the self-consistent Skyrme HF(+BCS)
+// iteration on a 3D lattice.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load this library, and only as the checker
+// or as the timed CPU baseline -- never as the product path.
+//
+// It restates, in plain C++17 + OpenMP (no Eigen), the algorithms of the
+// reference AlessandroSala/nuclearcgm (files cited per function, paths
+// relative to the reference root).  The reference itself needs Eigen, which is
+// not available here, so it cannot be built; Eigen's own routines used on the
+// path (ConjugateGradient with the diagonal preconditioner,
+// SelfAdjointEigenSolver reading the lower triangle) are restated from their
+// published algorithms.  BiCGSTAB for the Coulomb problem is replaced by CG on
+// the same symmetric operator to 1e-13 relative residual (same solution).
+//
+// Pinning (see tests/): polynomial exactness of every FD stencil, matrix-free
+// H == the CSR assembled element by element from the getElement5p rules,
+// constant-mass HO spectrum, dense brute-force eigenvalues vs GCG, Eigen-CG
+// iterates vs a textbook PCG, uniform-sphere Coulomb, constant-G BCS closed
+// form, EDF coefficient table, analytic-Gaussian densities and energies.
+
+#include "../include/nhf.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <tuple>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+using cd = std::complex<double>;
+using vecd = std::vector<double>;
+using vecc = std::vector<cd>;
+
+// include/constants.hpp:11-25
+constexpr double HBAR = 197.327053;
+constexpr double MNUC = 938.91869;
+constexpr double E2 = 1.43996446;
+const double PI = 3.14159265358979323846;
+
+// ------------------------------------------------------------------ grid
+// grid.cpp:7-41, grid.hpp:87-92
+struct Grid {
+  int n;
+  double a, h;
+  size_t N3;
+  vecd x;
+  Grid(int n_, double a_) : n(n_), a(a_), h(2.0 * a_ / (n_ - 1)), N3((size_t)n_ * n_ * n_), x(n_) {
+    for (int i = 0; i < n; ++i) x[i] = -a + i * h;
+  }
+  size_t idx(int i, int j, int k) const { return (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k); }
+  // trapezoid weight (integral_operators.cpp:62-87)
+  double w(int i, int j, int k) const {
+    double ww = 1.0;
+    if (i == 0 || i == n - 1) ww *= 0.5;
+    if (j == 0 || j == n - 1) ww *= 0.5;
+    if (k == 0 || k == n - 1) ww *= 0.5;
+    return ww;
+  }
+};
+
+static double integral(const Grid& g, const double* f) {
+  double hhh = g.h * g.h * g.h, res = 0.0;
+  for (int k = 0; k < g.n; ++k)
+    for (int j = 0; j < g.n; ++j)
+      for (int i = 0; i < g.n; ++i) res += f[g.idx(i, j, k)] * hhh * g.w(i, j, k);
+  return res;
+}
+static double integral(const Grid& g, const vecd& f) { return integral(g, f.data()); }
+
+// ------------------------------------------------------- FD derivatives
+// First derivative with the boundary degradation of
+// differential_operators.cpp:356-417 (real) and :482-543 (spinor):
+// 2-pt at 0 and n-1, 3-pt at 1 and n-2, 5-pt at 2 and n-3, 7-pt at 3 and
+// n-4, 9-pt (8th order) elsewhere.
+template <class T, class F>
+static T d1(F f, int pos, int n, double h) {
+  if (pos == 0) return (f(1) - f(0)) / h;
+  if (pos == 1 || pos == n - 2) return (f(pos + 1) - f(pos - 1)) / (2.0 * h);
+  if (pos == n - 1) return (f(pos) - f(pos - 1)) / h;
+  if (pos == 2 || pos == n - 3)
+    return (-f(pos + 2) + 8.0 * f(pos + 1) - 8.0 * f(pos - 1) + f(pos - 2)) / (12.0 * h);
+  if (pos == 3 || pos == n - 4)
+    return (-f(pos - 3) + 9.0 * f(pos - 2) - 45.0 * f(pos - 1) + 45.0 * f(pos + 1) - 9.0 * f(pos + 2) +
+            f(pos + 3)) /
+           (60.0 * h);
+  return (3.0 * f(pos - 4) - 32.0 * f(pos - 3) + 168.0 * f(pos - 2) - 672.0 * f(pos - 1) + 672.0 * f(pos + 1) -
+          168.0 * f(pos + 2) + 32.0 * f(pos + 3) - 3.0 * f(pos + 4)) /
+         (840.0 * h);
+}
+// Second derivative, differential_operators.cpp:545-610, including the quirk
+// that pos 0 and n-1 return a first-difference expression (:560-575).
+template <class T, class F>
+static T d2(F f, int pos, int n, double h) {
+  double hh = h * h;
+  if (pos == 0) return (f(1) - f(0)) / h;
+  if (pos == 1 || pos == n - 2) return (f(pos - 1) - 2.0 * f(pos) + f(pos + 1)) / hh;
+  if (pos == n - 1) return (f(pos) - f(pos - 1)) / h;
+  if (pos == 2 || pos == n - 3)
+    return (-f(pos + 2) + 16.0 * f(pos + 1) - 30.0 * f(pos) + 16.0 * f(pos - 1) - f(pos - 2)) / (12.0 * hh);
+  if (pos == 3 || pos == n - 4)
+    return (2.0 * f(pos + 3) - 27.0 * f(pos + 2) + 270.0 * f(pos + 1) - 490.0 * f(pos) + 270.0 * f(pos - 1) -
+            27.0 * f(pos - 2) + 2.0 * f(pos - 3)) /
+           (180.0 * hh);
+  return (-9.0 * f(pos - 4) + 128.0 * f(pos - 3) - 1008.0 * f(pos - 2) + 8064.0 * f(pos - 1) - 14350.0 * f(pos) +
+          8064.0 * f(pos + 1) - 1008.0 * f(pos + 2) + 128.0 * f(pos + 3) - 9.0 * f(pos + 4)) /
+         (5040.0 * hh);
+}
+
+static inline bool isnan_c(cd z) { return std::isnan(z.real()) || std::isnan(z.imag()); }
+
+// dvNoSpin (differential_operators.cpp:5-24): derivative of a scalar field
+// along axis, NaN -> 0.
+static void dv_real(const Grid& g, const double* f, int axis, double* out) {
+  int n = g.n;
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        int pos = axis == 0 ? i : axis == 1 ? j : k;
+        auto get = [&](int q) {
+          return axis == 0 ? f[g.idx(q, j, k)] : axis == 1 ? f[g.idx(i, q, k)] : f[g.idx(i, j, q)];
+        };
+        double v = d1<double>(get, pos, n, g.h);
+        out[g.idx(i, j, k)] = std::isnan(v) ? 0.0 : v;
+      }
+}
+static void d2v_real(const Grid& g, const double* f, int axis, double* out) {
+  int n = g.n;
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        int pos = axis == 0 ? i : axis == 1 ? j : k;
+        auto get = [&](int q) {
+          return axis == 0 ? f[g.idx(q, j, k)] : axis == 1 ? f[g.idx(i, q, k)] : f[g.idx(i, j, q)];
+        };
+        out[g.idx(i, j, k)] = d2<double>(get, pos, n, g.h);
+      }
+}
+// dv (differential_operators.cpp:82-107): derivative of a spinor column.
+static void dv_spinor(const Grid& g, const cd* psi, int axis, cd* out) {
+  int n = g.n;
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i)
+        for (int s = 0; s < 2; ++s) {
+          int pos = axis == 0 ? i : axis == 1 ? j : k;
+          auto get = [&](int q) {
+            size_t p = axis == 0 ? g.idx(q, j, k) : axis == 1 ? g.idx(i, q, k) : g.idx(i, j, q);
+            return psi[2 * p + s];
+          };
+          cd v = d1<cd>(get, pos, n, g.h);
+          out[2 * g.idx(i, j, k) + s] = isnan_c(v) ? cd(0.0, 0.0) : v;
+        }
+}
+
+// ------------------------------------------------------------------- EDF
+// EDF.cpp:59-98 (setFromInteraction).  p: t0 t1 t2 t3 x0 x1 x2 x3 W0 W0' te to sigma
+static void edf_from_skyrme(const double* p, double* C) {
+  double t0 = p[0], t1 = p[1], t2 = p[2], t3 = p[3], x0 = p[4], x1 = p[5], x2 = p[6], x3 = p[7];
+  double W0 = p[8], W0pr = p[9], te = p[10], to = p[11];
+  C[NHF_EDF_SIGMA] = p[12];
+  C[NHF_EDF_C0RR] = 3 * t0 / 8;
+  C[NHF_EDF_C1RR] = -t0 * (0.5 + x0) / 4;
+  C[NHF_EDF_C0DRR] = 3 * t3 / 48;
+  C[NHF_EDF_C1DRR] = -t3 * (0.5 + x3) / 24;
+  C[NHF_EDF_C0RT] = 3 * t1 / 16 + t2 * (5.0 / 4.0 + x2) / 4;
+  C[NHF_EDF_C1RT] = -t1 * (0.5 + x1) / 8 + t2 * (0.5 + x2) / 8;
+  C[NHF_EDF_C0RDR] = -9 * t1 / 64 + t2 * (5.0 / 4.0 + x2) / 16;
+  C[NHF_EDF_C1RDR] = 3 * t1 * (0.5 + x1) / 32 + t2 * (0.5 + x2) / 32;
+  C[NHF_EDF_C0NJ] = -0.5 * W0 - 0.25 * W0pr;
+  C[NHF_EDF_C1NJ] = -0.5 * W0;
+  double C0T = -(t1 * (1 - 2 * x1) - t2 * (1 + 2 * x2)) / 16;
+  double C1T = -(t1 - t2) / 16;
+  double C0F = -3 * (te + 3 * to) / 8;
+  double C1F = -3 * (te - to) / 8;
+  C[NHF_EDF_C0J2] = -0.5 * (C0T - 0.5 * C0F);
+  C[NHF_EDF_C1J2] = -0.5 * (C1T - 0.5 * C1F);
+}
+
+// --------------------------------------------------------- Hamiltonian
+// Single-particle fields of one species (what IterationData shares with the
+// potential terms): f = massq->vector, gf = massq->gradient, U = Uq
+// (+ UCoul for protons), W = Bq.
+struct Fields {
+  vecd f, gf, U, W;  // gf, W: 3 planes
+};
+
+// Matrix-free H.psi of one state column.  Restates the element rules of
+// LocalKineticPotential (local_kinetic_potential.cpp:22-46),
+// NonLocalKineticPotential (non_local_kinetic_potential.cpp:23-71),
+// SkyrmeU (skyrme_u.cpp:15-24), SkyrmeSO (skyrme_so.cpp:12-58) and
+// LocalCoulombPotential (local_coulomb_potential.cpp:13-41) summed by
+// Hamiltonian::buildMatrix (hamiltonian.cpp:303-372): every coefficient at
+// the row point, 5-point 4th-order stencils truncated at the box edge.
+static void hpsi_col(const Grid& g, const Fields& F, const cd* psi, cd* out) {
+  const int n = g.n;
+  const double h = g.h, hh = h * h;
+  const size_t N3 = g.N3;
+  const cd I(0.0, 1.0);
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        size_t p = g.idx(i, j, k);
+        int pos[3] = {i, j, k};
+        size_t stride[3] = {1, (size_t)n, (size_t)n * n};
+        cd lap[2] = {0.0, 0.0}, d[3][2];
+        for (int ax = 0; ax < 3; ++ax)
+          for (int s = 0; s < 2; ++s) {
+            auto get = [&](int q) -> cd {
+              int pp = pos[ax] + q;
+              if (pp < 0 || pp >= n) return cd(0.0, 0.0);
+              return psi[2 * (p + (long)q * (long)stride[ax]) + s];
+            };
+            lap[s] += (-(get(2) + get(-2)) + 16.0 * (get(1) + get(-1)) - 30.0 * get(0)) / (12.0 * hh);
+            d[ax][s] = (8.0 * (get(1) - get(-1)) - (get(2) - get(-2))) / (12.0 * h);
+          }
+        double f = F.f[p], U = F.U[p];
+        double gx = F.gf[p], gy = F.gf[N3 + p], gz = F.gf[2 * N3 + p];
+        double Wx = F.W[p], Wy = F.W[N3 + p], Wz = F.W[2 * N3 + p];
+        // (sigma x W)_a = S_a; H_SO psi_s = -i sum_a sum_s' (S_a)_{ss'} d_a psi_s'
+        cd T0 = -Wy * d[0][0] - I * Wz * d[0][1] + Wx * d[1][0] - Wz * d[1][1] + cd(Wy, Wx) * d[2][1];
+        cd T1 = I * Wz * d[0][0] + Wy * d[0][1] - Wz * d[1][0] - Wx * d[1][1] + cd(Wy, -Wx) * d[2][0];
+        for (int s = 0; s < 2; ++s) {
+          cd T = s == 0 ? T0 : T1;
+          out[2 * p + s] = -f * lap[s] - (gx * d[0][s] + gy * d[1][s] + gz * d[2][s]) + U * psi[2 * p + s] - I * T;
+        }
+      }
+}
+
+// Element-by-element assembly (the reference's pull interface), used only
+// to pin hpsi_col on tiny lattices.  Each term is the restated getElement5p.
+static cd elem_local_kinetic(const Grid& g, const Fields& F, int i, int j, int k, int s, int i1, int j1, int k1,
+                             int s1) {
+  double hh = g.h * g.h;
+  double C = -F.f[g.idx(i, j, k)];
+  double val = 0.0;
+  if (i == i1 && j == j1 && k == k1 && s == s1)
+    val = -C * (90.0 / 12.0) / hh;
+  else if (s == s1 && ((i == i1 && j == j1 && std::abs(k1 - k) == 1) || (i == i1 && k == k1 && std::abs(j1 - j) == 1) ||
+                       (j == j1 && k == k1 && std::abs(i1 - i) == 1)))
+    val = C * (16.0 / 12.0) / hh;
+  else if (s == s1 && ((i == i1 && j == j1 && std::abs(k1 - k) == 2) || (i == i1 && k == k1 && std::abs(j1 - j) == 2) ||
+                       (j == j1 && k == k1 && std::abs(i1 - i) == 2)))
+    val = -C * (1.0 / 12.0) / hh;
+  return cd(val, 0.0);
+}
+static cd elem_nonlocal_kinetic(const Grid& g, const Fields& F, int i, int j, int k, int s, int i1, int j1, int k1,
+                                int s1) {
+  if (i == i1 && j == j1 && k == k1 && s == s1) return 0.0;
+  if (s != s1) return 0.0;
+  size_t p = g.idx(i, j, k);
+  double C[3] = {-F.gf[p], -F.gf[g.N3 + p], -F.gf[2 * g.N3 + p]};
+  double val = 0.0;
+  if (i == i1 && j == j1 && std::abs(k1 - k) == 1)
+    val += (k1 - k) * 8.0 * C[2];
+  else if (i == i1 && k == k1 && std::abs(j1 - j) == 1)
+    val += (j1 - j) * 8.0 * C[1];
+  else if (j == j1 && k == k1 && std::abs(i1 - i) == 1)
+    val += (i1 - i) * 8.0 * C[0];
+  else if (i == i1 && j == j1 && std::abs(k1 - k) == 2)
+    val += -(k1 - k) * C[2] / 2.0;
+  else if (i == i1 && k == k1 && std::abs(j1 - j) == 2)
+    val += -(j1 - j) * C[1] / 2.0;
+  else if (j == j1 && k == k1 && std::abs(i1 - i) == 2)
+    val += -(i1 - i) * C[0] / 2.0;
+  return cd(val / (12.0 * g.h), 0.0);
+}
+static cd elem_u(const Grid& g, const Fields& F, int i, int j, int k, int s, int i1, int j1, int k1, int s1) {
+  if (i != i1 || j != j1 || k != k1 || s != s1) return 0.0;
+  return cd(F.U[g.idx(i, j, k)], 0.0);
+}
+typedef cd Mat2[2][2];
+static void pauli(int a, Mat2 m) {
+  const cd I(0, 1);
+  if (a == 0) { m[0][0] = 0; m[0][1] = 1; m[1][0] = 1; m[1][1] = 0; }
+  if (a == 1) { m[0][0] = 0; m[0][1] = -I; m[1][0] = I; m[1][1] = 0; }
+  if (a == 2) { m[0][0] = 1; m[0][1] = 0; m[1][0] = 0; m[1][1] = -1; }
+}
+static cd elem_so(const Grid& g, const Fields& F, int i, int j, int k, int s, int i1, int j1, int k1, int s1) {
+  if (i1 == i && j1 == j && k1 == k && s1 == s) return 0.0;
+  size_t p = g.idx(i, j, k);
+  double Wx = F.W[p], Wy = F.W[g.N3 + p], Wz = F.W[2 * g.N3 + p];
+  Mat2 sx, sy, sz;
+  pauli(0, sx);
+  pauli(1, sy);
+  pauli(2, sz);
+  cd spin[2][2] = {{0, 0}, {0, 0}};
+  auto acc = [&](double coef, double ca, Mat2 A, double cb, Mat2 B) {
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) spin[r][c] += coef * (ca * A[r][c] + cb * B[r][c]);
+  };
+  if (std::abs(i - i1) == 1 && j == j1 && k == k1)
+    acc((2.0 / 3.0) * (i1 - i), Wz, sy, -Wy, sz);
+  else if (std::abs(i - i1) == 2 && j == j1 && k == k1)
+    acc(-(i1 - i) * (1.0 / 24.0), Wz, sy, -Wy, sz);
+  else if (i == i1 && std::abs(j - j1) == 1 && k == k1)
+    acc((2.0 / 3.0) * (j1 - j), -Wz, sx, Wx, sz);
+  else if (i == i1 && std::abs(j - j1) == 2 && k == k1)
+    acc(-(j1 - j) * (1.0 / 24.0), -Wz, sx, Wx, sz);
+  else if (i == i1 && j == j1 && std::abs(k - k1) == 1)
+    acc((2.0 / 3.0) * (k1 - k), Wy, sx, -Wx, sy);
+  else if (i == i1 && j == j1 && std::abs(k - k1) == 2)
+    acc(-(k1 - k) * (1.0 / 24.0), Wy, sx, -Wx, sy);
+  else
+    return 0.0;
+  const cd I(0, 1);
+  return -I * spin[s][s1] / g.h;
+}
+
+// Hamiltonian::buildMatrix (hamiltonian.cpp:303-372) as CSR (rows sorted).
+static long build_csr(const Grid& g, const Fields& F, bool so, long* rowptr, int* col, cd* val, long cap) {
+  int n = g.n;
+  long nnz = 0;
+  size_t rows = 2 * g.N3;
+  std::vector<std::vector<std::pair<int, cd>>> R(rows);
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i)
+        for (int s = 0; s < 2; ++s)
+          for (int s1 = 0; s1 < 2; ++s1)
+            for (int k1 = std::max(0, k - 2); k1 <= std::min(n - 1, k + 2); ++k1)
+              for (int j1 = std::max(0, j - 2); j1 <= std::min(n - 1, j + 2); ++j1)
+                for (int i1 = std::max(0, i - 2); i1 <= std::min(n - 1, i + 2); ++i1) {
+                  cd v = elem_u(g, F, i, j, k, s, i1, j1, k1, s1) +
+                         elem_nonlocal_kinetic(g, F, i, j, k, s, i1, j1, k1, s1) +
+                         elem_local_kinetic(g, F, i, j, k, s, i1, j1, k1, s1);
+                  if (so) v += elem_so(g, F, i, j, k, s, i1, j1, k1, s1);
+                  if (v != cd(0.0, 0.0)) R[s + 2 * g.idx(i, j, k)].push_back({(int)(s1 + 2 * g.idx(i1, j1, k1)), v});
+                }
+  rowptr[0] = 0;
+  for (size_t r = 0; r < rows; ++r) {
+    std::sort(R[r].begin(), R[r].end(), [](auto& x, auto& y) { return x.first < y.first; });
+    for (auto& e : R[r]) {
+      if (nnz < cap) {
+        col[nnz] = e.first;
+        val[nnz] = e.second;
+      }
+      ++nnz;
+    }
+    rowptr[r + 1] = nnz;
+  }
+  return nnz;
+}
+
+// Operator object used by the solvers: H (+ shift), diagonal for Jacobi.
+struct Op {
+  const Grid* g;
+  const Fields* F;
+  double shift = 0.0;
+  void apply(const cd* x, cd* y) const {
+    hpsi_col(*g, *F, x, y);
+    if (shift != 0.0) {
+      size_t rows = 2 * g->N3;
+      for (size_t r = 0; r < rows; ++r) y[r] += shift * x[r];
+    }
+  }
+  // diagonal element of row r: the LocalKinetic diagonal + U (+ shift);
+  // the non-local kinetic and spin-orbit terms vanish on the diagonal.
+  double diag(size_t r) const {
+    size_t p = r / 2;
+    return F->f[p] * (90.0 / 12.0) / (g->h * g->h) + F->U[p] + shift;
+  }
+};
+
+// -------------------------------------------------------- dense linear algebra
+static inline cd dotc(const cd* a, const cd* b, size_t n) {  // a^H b
+  cd s = 0.0;
+  for (size_t i = 0; i < n; ++i) s += std::conj(a[i]) * b[i];
+  return s;
+}
+static inline double nrm2sq(const cd* a, size_t n) {
+  double s = 0.0;
+  for (size_t i = 0; i < n; ++i) s += std::norm(a[i]);
+  return s;
+}
+
+// Hermitian eigensolver reading only the lower triangle (as Eigen's
+// SelfAdjointEigenSolver), ascending eigenvalues.  Cyclic complex Jacobi
+// rotations (an independent method from the library's Householder+QL):
+// A (k x k, column-major) in; w (k) and V (k x k, column-major) out.
+static bool eigh_lower(int k, const cd* Ain, double* w, cd* V) {
+  std::vector<cd> A((size_t)k * k);
+  auto a = [&](int r, int c) -> cd& { return A[(size_t)c * k + r]; };
+  for (int c = 0; c < k; ++c)
+    for (int r = 0; r < k; ++r) {
+      if (r > c)
+        a(r, c) = Ain[(size_t)c * k + r];
+      else if (r < c)
+        a(r, c) = std::conj(Ain[(size_t)r * k + c]);
+      else
+        a(r, c) = cd(Ain[(size_t)c * k + r].real(), 0.0);
+    }
+  for (int c = 0; c < k; ++c)
+    for (int r = 0; r < k; ++r) V[(size_t)c * k + r] = (r == c) ? 1.0 : 0.0;
+  auto v = [&](int r, int c) -> cd& { return V[(size_t)c * k + r]; };
+  for (int sweep = 0; sweep < 200; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int c = 0; c < k; ++c)
+      for (int r = 0; r < k; ++r) {
+        double m2 = std::norm(a(r, c));
+        tot += m2;
+        if (r != c) off += m2;
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < k - 1; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        cd apq = a(p, q);
+        double mag = std::abs(apq);
+        if (mag == 0.0) continue;
+        cd ph = apq / mag;  // e^{i phi}
+        double app = a(p, p).real(), aqq = a(q, q).real();
+        double theta = (aqq - app) / (2.0 * mag);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + std::sqrt(theta * theta + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        // G = D R: columns p,q of the unitary; G_pp = c, G_pq = s,
+        // G_qp = -s e^{-i phi}, G_qq = c e^{-i phi}
+        cd gqp = -s * std::conj(ph), gqq = c * std::conj(ph);
+        for (int r = 0; r < k; ++r) {  // A <- A G
+          cd arp = a(r, p), arq = a(r, q);
+          a(r, p) = arp * c + arq * gqp;
+          a(r, q) = arp * s + arq * gqq;
+        }
+        for (int r = 0; r < k; ++r) {  // A <- G^H A
+          cd apr = a(p, r), aqr = a(q, r);
+          a(p, r) = c * apr + std::conj(gqp) * aqr;
+          a(q, r) = s * apr + std::conj(gqq) * aqr;
+        }
+        a(p, q) = 0.0;
+        a(q, p) = 0.0;
+        a(p, p) = cd(a(p, p).real(), 0.0);
+        a(q, q) = cd(a(q, q).real(), 0.0);
+        for (int r = 0; r < k; ++r) {  // V <- V G
+          cd vrp = v(r, p), vrq = v(r, q);
+          v(r, p) = vrp * c + vrq * gqp;
+          v(r, q) = vrp * s + vrq * gqq;
+        }
+      }
+  }
+  std::vector<int> ord(k);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::vector<double> d(k);
+  for (int i = 0; i < k; ++i) d[i] = a(i, i).real();
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return d[x] < d[y]; });
+  std::vector<cd> Vs((size_t)k * k);
+  for (int c = 0; c < k; ++c) {
+    w[c] = d[ord[c]];
+    for (int r = 0; r < k; ++r) Vs[(size_t)c * k + r] = V[(size_t)ord[c] * k + r];
+  }
+  std::copy(Vs.begin(), Vs.end(), V);
+  for (int i = 0; i < k; ++i)
+    if (!std::isfinite(w[i])) return false;
+  return true;
+}
+
+// Real symmetric eigensolver (lower triangle), for the HFB matrices.
+static bool eigh_real(int k, const double* A, double* w, double* V) {
+  std::vector<cd> Ac((size_t)k * k), Vc((size_t)k * k);
+  for (size_t i = 0; i < (size_t)k * k; ++i) Ac[i] = A[i];
+  bool ok = eigh_lower(k, Ac.data(), w, Vc.data());
+  // Jacobi on a real matrix keeps the vectors real up to rounding
+  for (size_t i = 0; i < (size_t)k * k; ++i) V[i] = Vc[i].real();
+  return ok;
+}
+
+// ---------------------------------------------------------- Eigen CG
+// Eigen::ConjugateGradient<..., Lower|Upper> with DiagonalPreconditioner,
+// solveWithGuess (solver.cpp:733-743): textbook preconditioned CG with the
+// stopping rule |r|^2 < max(tol^2 |b|^2, DBL_MIN) checked after each update
+// and at most maxIters matrix products; alpha = (r^H z) / (p^H A p) is complex
+// because the row-point Hamiltonian is not exactly Hermitian.
+static int eigen_cg(const Op& A, const cd* b, cd* x, int maxIters, double tol) {
+  size_t nr = 2 * A.g->N3;
+  vecc r(nr), p(nr), z(nr), tmp(nr);
+  A.apply(x, tmp.data());
+  for (size_t i = 0; i < nr; ++i) r[i] = b[i] - tmp[i];
+  double rhsNorm2 = nrm2sq(b, nr);
+  if (rhsNorm2 == 0) {
+    std::fill(x, x + nr, cd(0.0, 0.0));
+    return 0;
+  }
+  double threshold = std::max(tol * tol * rhsNorm2, 2.2250738585072014e-308);
+  double residualNorm2 = nrm2sq(r.data(), nr);
+  if (residualNorm2 < threshold) return 0;
+  vecd invdiag(nr);
+  for (size_t i = 0; i < nr; ++i) {
+    double dg = A.diag(i);
+    invdiag[i] = dg != 0.0 ? 1.0 / dg : 1.0;
+  }
+  for (size_t i = 0; i < nr; ++i) p[i] = r[i] * invdiag[i];
+  double absNew = dotc(r.data(), p.data(), nr).real();
+  int it = 0;
+  while (it < maxIters) {
+    A.apply(p.data(), tmp.data());
+    cd alpha = absNew / dotc(p.data(), tmp.data(), nr);
+    for (size_t i = 0; i < nr; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * tmp[i];
+    }
+    residualNorm2 = nrm2sq(r.data(), nr);
+    if (residualNorm2 < threshold) break;
+    for (size_t i = 0; i < nr; ++i) z[i] = r[i] * invdiag[i];
+    double absOld = absNew;
+    absNew = dotc(r.data(), z.data(), nr).real();
+    double beta = absNew / absOld;
+    for (size_t i = 0; i < nr; ++i) p[i] = z[i] + beta * p[i];
+    ++it;
+  }
+  return it;
+}
+
+// b_modified_gram_schmidt_complex_no_B (solver.cpp:389-416)
+static void mgs(vecc& V, size_t nr, int cols) {
+  for (int j = 0; j < cols; ++j) {
+    cd* vj = &V[(size_t)j * nr];
+    double nsq = dotc(vj, vj, nr).real();
+    double nb = (nsq > 1e-24) ? std::sqrt(nsq) : 0.0;
+    if (nb < 1e-12) {
+      std::fill(vj, vj + nr, cd(0.0, 0.0));
+      continue;
+    }
+    for (size_t i = 0; i < nr; ++i) vj[i] /= nb;
+#pragma omp parallel for
+    for (int k = j + 1; k < cols; ++k) {
+      cd* vk = &V[(size_t)k * nr];
+      cd proj = dotc(vj, vk, nr);
+      for (size_t i = 0; i < nr; ++i) vk[i] -= proj * vj[i];
+    }
+  }
+}
+
+// V^H (A V): k x k, column-major
+static void project(const Op& A, const vecc& V, size_t nr, int k, vecc& AV, vecc& P) {
+  AV.resize((size_t)k * nr);
+  for (int c = 0; c < k; ++c) A.apply(&V[(size_t)c * nr], &AV[(size_t)c * nr]);
+  P.assign((size_t)k * k, 0.0);
+#pragma omp parallel for collapse(2)
+  for (int c = 0; c < k; ++c)
+    for (int r = 0; r < k; ++r) P[(size_t)c * k + r] = dotc(&V[(size_t)r * nr], &AV[(size_t)c * nr], nr);
+}
+
+// Y = V C (V: nr x k, C: k x m), column-major
+static void rotate(const vecc& V, size_t nr, int k, const cd* C, int m, vecc& Y) {
+  Y.assign((size_t)m * nr, 0.0);
+#pragma omp parallel for
+  for (long i = 0; i < (long)nr; ++i)
+    for (int c = 0; c < m; ++c) {
+      cd s = 0.0;
+      for (int l = 0; l < k; ++l) s += V[(size_t)l * nr + i] * C[(size_t)c * k + l];
+      Y[(size_t)c * nr + i] = s;
+    }
+}
+
+// gcgm_complex_no_B_lock (solver.cpp:642-959), ASCENDING_ENERGIES ordering
+// (utilities.cpp:30-36).  X_init: nr x nev (at least nev columns), P0: conj.
+// directions (pc columns).  Returns false on eigensolver failure.
+struct GcgStats {
+  int cg_iters_total = 0;
+  int basis = 0;
+};
+static bool gcg(const Grid& g, const Fields& F, const vecc& X_init, int nev, const vecc& P0, int pc,
+                const nhf_gcg_params& prm, vecc& Xout, vecd& lam_out, GcgStats* st = nullptr) {
+  size_t nr = 2 * g.N3;
+  Op A{&g, &F, 0.0};
+  vecc X(X_init.begin(), X_init.begin() + (size_t)nev * nr);
+  vecc X0;  // converged (locked)
+  int nconv = 0;
+  vecc P = P0;
+  int pcols = pc;
+  mgs(X, nr, nev);
+  // initial Rayleigh-Ritz with shift 0 (rayleighRitz_complex_no_B, :418-463)
+  vecc AV, Pr;
+  project(A, X, nr, nev, AV, Pr);
+  vecd w(nev);
+  vecc C((size_t)nev * nev);
+  if (!eigh_lower(nev, Pr.data(), w.data(), C.data())) return false;
+  vecc Xr;
+  rotate(X, nr, nev, C.data(), nev, Xr);
+  X.swap(Xr);
+  vecd Lambda = w;
+  int nact = nev;
+  double shift = 0.0;
+  for (int iter = 0; iter < prm.max_iter; ++iter) {
+    if (nconv == nev) break;
+    if ((int)Lambda.size() > 1) shift = (Lambda[Lambda.size() - 1] - 100.0 * Lambda[0]) / 99.0;
+    Op As{&g, &F, shift};
+    vecc W((size_t)nact * nr);
+    std::vector<int> its(nact, 0);
+    {
+#pragma omp parallel for schedule(dynamic)
+      for (int c = 0; c < nact; ++c) {
+        vecc b(nr);
+        const cd* xc = &X[(size_t)c * nr];
+        for (size_t i = 0; i < nr; ++i) b[i] = xc[i] * (Lambda[c] + shift);
+        cd* wc = &W[(size_t)c * nr];
+        std::copy(xc, xc + nr, wc);
+        its[c] = eigen_cg(As, b.data(), wc, prm.steps, prm.cg_tol);
+      }
+    }
+    if (st)
+      for (int c = 0; c < nact; ++c) st->cg_iters_total += its[c];
+    int kk = nconv + nact + pcols + nact;
+    vecc V((size_t)kk * nr);
+    std::copy(X0.begin(), X0.end(), V.begin());
+    std::copy(X.begin(), X.end(), V.begin() + (size_t)nconv * nr);
+    if (pcols > 0) std::copy(P.begin(), P.begin() + (size_t)pcols * nr, V.begin() + (size_t)(nconv + nact) * nr);
+    std::copy(W.begin(), W.end(), V.begin() + (size_t)(nconv + nact + pcols) * nr);
+    mgs(V, nr, kk);
+    std::vector<int> keep;
+    for (int c = 0; c < kk; ++c)
+      if (nrm2sq(&V[(size_t)c * nr], nr) > 1e-20) keep.push_back(c);
+    int ke = (int)keep.size();
+    vecc Ve((size_t)ke * nr);
+    for (int c = 0; c < ke; ++c)
+      std::copy(V.begin() + (size_t)keep[c] * nr, V.begin() + (size_t)(keep[c] + 1) * nr, Ve.begin() + (size_t)c * nr);
+    if (st) st->basis = ke;
+    project(As, Ve, nr, ke, AV, Pr);
+    vecd wk(ke);
+    vecc Ck((size_t)ke * ke);
+    if (!eigh_lower(ke, Pr.data(), wk.data(), Ck.data())) return false;
+    int rr = std::min(nev, ke);
+    vecc Xnew;
+    rotate(Ve, nr, ke, Ck.data(), rr, Xnew);
+    // conjugate direction bookkeeping (P is only carried inside a call)
+    if (iter == prm.max_iter - 1) {
+      pcols = 0;
+      P.clear();
+    } else {
+      pcols = 0;
+      P.clear();
+    }
+    vecd Lnew(rr);
+    for (int i = 0; i < rr; ++i) Lnew[i] = wk[i] - shift;
+    // residuals A X - X Lambda with the unshifted operator
+    std::vector<double> rn(rr, 0.0);
+#pragma omp parallel for schedule(dynamic)
+    for (int c = nconv; c < rr; ++c) {
+      vecc ax(nr);
+      hpsi_col(g, F, &Xnew[(size_t)c * nr], ax.data());
+      double s = 0.0;
+      for (size_t i = 0; i < nr; ++i) s += std::norm(ax[i] - Lnew[c] * Xnew[(size_t)c * nr + i]);
+      rn[c] = std::sqrt(s);
+    }
+    vecc X0new(Xnew.begin(), Xnew.begin() + (size_t)nconv * nr);
+    std::vector<int> newc, act;
+    for (int i = nconv; i < rr; ++i) {
+      if (rn[i] < prm.tol && (nconv + (int)newc.size() < nev))
+        newc.push_back(i);
+      else
+        act.push_back(i);
+    }
+    X0 = X0new;
+    for (int c : newc) X0.insert(X0.end(), Xnew.begin() + (size_t)c * nr, Xnew.begin() + (size_t)(c + 1) * nr);
+    nconv += (int)newc.size();
+    X.clear();
+    Lambda.clear();
+    for (int c : act) {
+      X.insert(X.end(), Xnew.begin() + (size_t)c * nr, Xnew.begin() + (size_t)(c + 1) * nr);
+      Lambda.push_back(Lnew[c]);
+    }
+    nact = (int)act.size();
+  }
+  // final assembly (:887-955): locked columns get Rayleigh quotients
+  int take = std::min(nact, nev - nconv);
+  vecc FX((size_t)nev * nr, 0.0);
+  vecd FL(nev, 0.0);
+  std::copy(X0.begin(), X0.end(), FX.begin());
+  for (int c = 0; c < take; ++c)
+    std::copy(X.begin() + (size_t)c * nr, X.begin() + (size_t)(c + 1) * nr, FX.begin() + (size_t)(nconv + c) * nr);
+  for (int c = 0; c < nconv; ++c) {
+    vecc ax(nr);
+    hpsi_col(g, F, &X0[(size_t)c * nr], ax.data());
+    FL[c] = dotc(&X0[(size_t)c * nr], ax.data(), nr).real();
+  }
+  for (int c = 0; c < take; ++c) FL[nconv + c] = Lambda[c];
+  std::vector<int> ord(nev);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return FL[a] < FL[b]; });
+  Xout.assign((size_t)nev * nr, 0.0);
+  lam_out.assign(nev, 0.0);
+  for (int c = 0; c < nev; ++c) {
+    lam_out[c] = FL[ord[c]];
+    std::copy(FX.begin() + (size_t)ord[c] * nr, FX.begin() + (size_t)(ord[c] + 1) * nr, Xout.begin() + (size_t)c * nr);
+  }
+  return true;
+}
+
+// ------------------------------------------------------------- densities
+// Wavefunction::density / kineticDensity / soDensity / divJ
+// (wavefunction.cpp:21-44, 348-360, 421-464, 466-517) for occupation
+// weights occ[c] = v^2 (the reference scales psi by sqrt(v^2)).
+struct Dens {
+  vecd rho, tau, J, divJ;  // J: 9 planes, column mu*3+nu
+};
+static void densities(const Grid& g, const vecc& psi, int m, const double* occ, Dens& D) {
+  size_t N3 = g.N3, nr = 2 * N3;
+  D.rho.assign(N3, 0.0);
+  D.tau.assign(N3, 0.0);
+  D.J.assign(9 * N3, 0.0);
+  D.divJ.assign(N3, 0.0);
+  vecc grad(3 * nr);
+  for (int c = 0; c < m; ++c) {
+    double wgt = occ[c];
+    if (wgt == 0.0) continue;
+    const cd* ps = &psi[(size_t)c * nr];
+    for (int ax = 0; ax < 3; ++ax) dv_spinor(g, ps, ax, &grad[ax * nr]);
+#pragma omp parallel for
+    for (long p = 0; p < (long)N3; ++p) {
+      cd u = ps[2 * p], d = ps[2 * p + 1];
+      D.rho[p] += wgt * (std::norm(u) + std::norm(d));
+      double t = 0.0;
+      for (int ax = 0; ax < 3; ++ax) t += std::norm(grad[ax * nr + 2 * p]) + std::norm(grad[ax * nr + 2 * p + 1]);
+      D.tau[p] += wgt * t;
+      // J_{mu nu} = Im[ psi^dag sigma_nu d_mu psi ]
+      for (int mu = 0; mu < 3; ++mu) {
+        cd g0 = grad[mu * nr + 2 * p], g1 = grad[mu * nr + 2 * p + 1];
+        for (int nu = 0; nu < 3; ++nu) {
+          Mat2 s;
+          pauli(nu, s);
+          cd pr0 = s[0][0] * g0 + s[0][1] * g1, pr1 = s[1][0] * g0 + s[1][1] * g1;
+          D.J[(mu * 3 + nu) * N3 + p] += wgt * (std::conj(u) * pr0 + std::conj(d) * pr1).imag();
+        }
+      }
+      // div J = -Im sum eps_{mu nu la} (d_nu psi)^dag sigma_la (d_mu psi)
+      cd sum = 0.0;
+      static const int E[6][4] = {{0, 1, 2, 1}, {1, 2, 0, 1}, {2, 0, 1, 1}, {2, 1, 0, -1}, {0, 2, 1, -1}, {1, 0, 2, -1}};
+      for (auto& e : E) {
+        int mu = e[0], nu = e[1], la = e[2];
+        Mat2 s;
+        pauli(la, s);
+        cd m0 = grad[mu * nr + 2 * p], m1 = grad[mu * nr + 2 * p + 1];
+        cd n0 = grad[nu * nr + 2 * p], n1 = grad[nu * nr + 2 * p + 1];
+        cd term = std::conj(n0) * (s[0][0] * m0 + s[0][1] * m1) + std::conj(n1) * (s[1][0] * m0 + s[1][1] * m1);
+        sum += double(e[3]) * term;
+      }
+      D.divJ[p] += -wgt * sum.imag();
+    }
+  }
+}
+
+// Operators::leviCivita (common_operators.cpp:24-38)
+static void levi_civita(size_t N3, const vecd& J, vecd& Jv) {
+  Jv.assign(3 * N3, 0.0);
+  for (size_t p = 0; p < N3; ++p) {
+    Jv[p] = J[(1 * 3 + 2) * N3 + p] - J[(2 * 3 + 1) * N3 + p];
+    Jv[N3 + p] = J[(2 * 3 + 0) * N3 + p] - J[(0 * 3 + 2) * N3 + p];
+    Jv[2 * N3 + p] = J[(0 * 3 + 1) * N3 + p] - J[(1 * 3 + 0) * N3 + p];
+  }
+}
+
+// --------------------------------------------------------------- Coulomb
+// coulombFieldPoisson (wavefunction.cpp:55-229).  Ghost-shell boundary
+// values e^2 Z / r + e^2 (Y20 Q20 + 2 Y22 Q22) / r are read, as in the
+// reference, through an x-slowest index while the right-hand side uses the
+// x-fastest grid index: the ghost neighbour (X,Y,Z) of the lattice receives
+// the value at coordinates (c(Z), c(Y), c(X)).  The operator is the 5-point
+// 4th-order Laplacian truncated at the box (laplacianp_potential.cpp:18-73).
+static void lap4(const Grid& g, const double* u, double* out) {
+  int n = g.n;
+  double hh = g.h * g.h;
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        int pos[3] = {i, j, k};
+        size_t st[3] = {1, (size_t)n, (size_t)n * n};
+        size_t p = g.idx(i, j, k);
+        double s = -(90.0 / 12.0) * u[p];
+        for (int ax = 0; ax < 3; ++ax)
+          for (int d = -2; d <= 2; ++d) {
+            if (d == 0) continue;
+            int q = pos[ax] + d;
+            if (q < 0 || q >= n) continue;
+            double c = (std::abs(d) == 1) ? 16.0 / 12.0 : -1.0 / 12.0;
+            s += c * u[p + (long)d * (long)st[ax]];
+          }
+        out[p] = s / hh;
+      }
+}
+static double Y20c(double x, double y, double z) {
+  double th = std::acos(z / std::sqrt(x * x + y * y + z * z));
+  return std::sqrt(5.0 / (16.0 * PI)) * (3.0 * std::cos(th) * std::cos(th) - 1.0);
+}
+static double Y22c(double x, double y, double z) {
+  double th = std::acos(z / std::sqrt(x * x + y * y + z * z));
+  double ph = std::atan2(y, x);
+  return std::sqrt(15.0 / (32.0 * PI)) * std::cos(2 * ph) * std::sin(th) * std::sin(th);
+}
+static void coulomb_rhs(const Grid& g, const double* rhoP, int Z, vecd& rhs) {
+  int n = g.n;
+  size_t N3 = g.N3;
+  vecd q20(N3), q22(N3);
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        size_t p = g.idx(i, j, k);
+        double x = g.x[i], y = g.x[j], z = g.x[k];
+        q20[p] = rhoP[p] * (2.0 * z * z - x * x - y * y) * std::sqrt(5.0 / (16.0 * PI));
+        q22[p] = rhoP[p] * (x * x - y * y) * std::sqrt(15.0 / (32.0 * PI));
+      }
+  double Q20 = integral(g, q20), Q22 = integral(g, q22);
+  auto c = [&](int t) { return -g.a + t * g.h; };
+  auto ubc = [&](double x, double y, double z) {
+    double r = std::sqrt(x * x + y * y + z * z);
+    if (r < 1e-9) r = 1e-9;
+    return (E2 * Z) / r + E2 * (Y20c(x, y, z) * Q20 + 2.0 * Y22c(x, y, z) * Q22) / r;
+  };
+  auto ghost = [&](int X, int Y, int Zc) { return ubc(c(Zc), c(Y), c(X)); };
+  double hh = g.h * g.h;
+  rhs.assign(N3, 0.0);
+  for (size_t p = 0; p < N3; ++p) rhs[p] = -(E2 * 4 * PI) * rhoP[p];
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        int pos[3] = {i, j, k};
+        size_t p = g.idx(i, j, k);
+        for (int ax = 0; ax < 3; ++ax)
+          for (int d = -2; d <= 2; ++d) {
+            if (d == 0) continue;
+            int q = pos[ax] + d;
+            if (q >= 0 && q < n) continue;
+            int G[3] = {i, j, k};
+            G[ax] = q;
+            double cf = (std::abs(d) == 1) ? 16.0 / 12.0 : -1.0 / 12.0;
+            rhs[p] -= cf * ghost(G[0], G[1], G[2]) / hh;
+          }
+      }
+}
+// CG on (-Lap) U = -rhs, warm start U.
+static int poisson(const Grid& g, const double* rhoP, int Z, double* U, double tol, int maxit) {
+  size_t N3 = g.N3;
+  vecd rhs;
+  coulomb_rhs(g, rhoP, Z, rhs);
+  vecd r(N3), p(N3), Ap(N3);
+  lap4(g, U, Ap.data());
+  double bn = 0, rr = 0;
+  for (size_t i = 0; i < N3; ++i) {
+    r[i] = -rhs[i] + Ap[i];  // b - A x with A = -Lap, b = -rhs
+    bn += rhs[i] * rhs[i];
+    rr += r[i] * r[i];
+  }
+  p = r;
+  int it = 0;
+  while (it < maxit && rr > tol * tol * bn) {
+    lap4(g, p.data(), Ap.data());
+    double pap = 0;
+    for (size_t i = 0; i < N3; ++i) pap += -p[i] * Ap[i];
+    double al = rr / pap, rr2 = 0;
+    for (size_t i = 0; i < N3; ++i) {
+      U[i] += al * p[i];
+      r[i] -= al * (-Ap[i]);
+      rr2 += r[i] * r[i];
+    }
+    double be = rr2 / rr;
+    rr = rr2;
+    for (size_t i = 0; i < N3; ++i) p[i] = r[i] + be * p[i];
+    ++it;
+  }
+  return it;
+}
+
+// ----------------------------------------------------------- time reversal
+// Wavefunction::timeReverse (wavefunction.cpp:554-580): T psi = i sigma_y K psi
+static void time_reverse(size_t N3, const cd* psi, cd* out) {
+  for (size_t p = 0; p < N3; ++p) {
+    cd u = psi[2 * p], d = psi[2 * p + 1];
+    out[2 * p] = std::conj(d);
+    out[2 * p + 1] = -std::conj(u);
+  }
+}
+// O_kj = psi_k^H (T psi_j)
+static void tr_overlap(size_t N3, const vecc& psi, int M, vecc& O) {
+  size_t nr = 2 * N3;
+  vecc T((size_t)M * nr);
+  for (int c = 0; c < M; ++c) time_reverse(N3, &psi[(size_t)c * nr], &T[(size_t)c * nr]);
+  O.assign((size_t)M * M, 0.0);
+#pragma omp parallel for collapse(2)
+  for (int j = 0; j < M; ++j)
+    for (int k = 0; k < M; ++k) O[(size_t)j * M + k] = dotc(&psi[(size_t)k * nr], &T[(size_t)j * nr], nr);
+}
+// Wavefunction::TROrder (wavefunction.cpp:581-636)
+static void tr_order(size_t N3, vecc& psi, int M) {
+  size_t nr = 2 * N3;
+  vecc O;
+  tr_overlap(N3, psi, M, O);
+  vecc out((size_t)M * nr);
+  std::vector<bool> used(M, false);
+  int slot = 0;
+  for (int i = 0; i < M; ++i) {
+    if (used[i]) continue;
+    std::copy(psi.begin() + (size_t)i * nr, psi.begin() + (size_t)(i + 1) * nr, out.begin() + (size_t)slot * nr);
+    used[i] = true;
+    int best = -1;
+    double mx = -1.0;
+    for (int j = 0; j < M; ++j) {
+      if (used[j]) continue;
+      double ov = std::abs(O[(size_t)j * M + i]);
+      if (ov > mx) {
+        mx = ov;
+        best = j;
+      }
+    }
+    if (best == -1) return;  // error path: keep the input order
+    std::copy(psi.begin() + (size_t)best * nr, psi.begin() + (size_t)(best + 1) * nr,
+              out.begin() + (size_t)(slot + 1) * nr);
+    used[best] = true;
+    slot += 2;
+    if (slot >= M) break;
+  }
+  psi.swap(out);
+}
+
+// ------------------------------------------------------------------- BCS
+// bcs.cpp:65-321
+struct BCSRes {
+  vecd u2, v2, Delta, qp;
+  double lambda = 0.0, Epair = 0.0;
+};
+static std::vector<int> match_time_reversal(const vecc& O, int M, double threshold = 1e-6) {
+  std::vector<std::tuple<double, int, int>> list;
+  for (int k = 0; k < M; ++k)
+    for (int j = 0; j < M; ++j) {
+      double v = std::norm(O[(size_t)j * M + k]);  // pairs() = |O|^2
+      if (v > threshold) list.emplace_back(v, k, j);
+    }
+  std::stable_sort(list.begin(), list.end(), [](auto& a, auto& b) { return std::get<0>(a) > std::get<0>(b); });
+  std::vector<int> partner(M, -1);
+  std::vector<char> used(M, 0);
+  for (auto& e : list) {
+    int k = std::get<1>(e), j = std::get<2>(e);
+    if (used[k] || used[j] || k == j) continue;
+    partner[k] = j;
+    partner[j] = k;
+    used[k] = used[j] = 1;
+  }
+  return partner;
+}
+// solveBCS (bcs.cpp:107-254)
+static BCSRes solve_bcs(const vecd& eps, const vecd& Gp, int A, const nhf_pairing& prm, const vecd& oldDelta,
+                        double oldLambda) {
+  const double EPS_SMALL = 1e-12, tol = 1e-10, prec = 1e-10;
+  int P = (int)eps.size();
+  BCSRes R;
+  if (P == 0) return R;
+  vecd G(Gp.size());
+  for (size_t i = 0; i < G.size(); ++i) G[i] = 0.5 * Gp[i];
+  vecd Delta(P, 1.0);
+  if ((int)oldDelta.size() == P && *std::max_element(oldDelta.begin(), oldDelta.end()) > 1e-3) Delta = oldDelta;
+  double ws = prm.window / 10.0;
+  double lambda = oldLambda;
+  if (!std::isfinite(lambda)) lambda = eps[std::clamp((A / 2) - 1, 0, P - 1)];
+  vecd ww(P), E(P);
+  auto window = [&](double lam) {
+    for (int p = 0; p < P; ++p) {
+      ww[p] = 1.0 / (1.0 + std::exp((eps[p] - lam - prm.window) / ws));
+      if (prm.window_both) ww[p] *= 1.0 / (1.0 + std::exp(-(eps[p] - lam + prm.window) / ws));
+    }
+  };
+  for (int iter = 0; iter < 2000; ++iter) {
+    double prevLambda = lambda;
+    window(lambda);
+    for (int p = 0; p < P; ++p) {
+      double xi = eps[p] - lambda;
+      E[p] = std::sqrt(xi * xi + Delta[p] * Delta[p] * ww[p]);
+    }
+    double s1 = 0, sInv = 0;
+    for (int p = 0; p < P; ++p) {
+      s1 += 1.0 - eps[p] / E[p];
+      sInv += 1.0 / E[p];
+    }
+    double raw = (sInv < EPS_SMALL) ? prevLambda : ((double)A - s1) / sInv;
+    lambda = 0.9 * raw + 0.1 * lambda;
+    vecd DN(P, 0.0);
+    for (int p = 0; p < P; ++p) {
+      double s = 0;
+      for (int q = 0; q < P; ++q)
+        if (ww[q] > EPS_SMALL) s += G[(size_t)q * P + p] * ww[q] * Delta[q] / E[q];
+      DN[p] = s;
+    }
+    double dn = 0;
+    for (int p = 0; p < P; ++p) {
+      double mixed = 0.1 * Delta[p] + 0.9 * DN[p];
+      dn += (mixed - Delta[p]) * (mixed - Delta[p]);
+      Delta[p] = mixed;
+    }
+    dn = std::sqrt(dn);
+    if (std::abs(raw - prevLambda) < prec && dn < tol) break;
+  }
+  window(lambda);
+  R.u2.resize(P);
+  R.v2.resize(P);
+  for (int i = 0; i < P; ++i) {
+    double xi = eps[i] - lambda;
+    E[i] = std::sqrt(xi * xi + Delta[i] * Delta[i] * ww[i]);
+    double v2 = std::clamp(0.5 * (1 - xi / E[i]), 0.0, 1.0);
+    R.v2[i] = v2;
+    R.u2[i] = 1 - v2;
+  }
+  double Ep = 0;
+  for (int p = 0; p < P; ++p)
+    for (int q = 0; q < P; ++q)
+      if (ww[q] > EPS_SMALL) Ep += -0.5 * ww[q] * ww[p] * Delta[p] * Delta[q] * G[(size_t)q * P + p] / E[p] / E[q];
+  R.Delta = Delta;
+  R.lambda = lambda;
+  R.Epair = Ep;
+  return R;
+}
+// BCS::BCSiter (bcs.cpp:256-321) with compute_G_pairing (:10-63).
+static BCSRes bcs_iter(const Grid& g, const vecc& psi, const vecd& eps, int M, int A, const nhf_pairing& prm,
+                       const vecd& oldDelta, double oldLambda) {
+  size_t N3 = g.N3, nr = 2 * N3;
+  vecc O;
+  tr_overlap(N3, psi, M, O);
+  auto partner = match_time_reversal(O, M);
+  std::vector<int> uniq, s2p(M, 0);
+  std::vector<bool> vis(M, false);
+  for (int i = 0; i < M; ++i) {
+    if (vis[i]) continue;
+    int pt = partner[i];
+    if (pt != i && pt != -1) {
+      int pi = (int)uniq.size();
+      uniq.push_back(i);
+      s2p[i] = pi;
+      s2p[pt] = pi;
+      vis[i] = vis[pt] = true;
+    }
+  }
+  int P = (int)uniq.size();
+  vecd ep(P);
+  for (int p = 0; p < P; ++p) ep[p] = eps[uniq[p]];
+  vecd odu;
+  if ((int)oldDelta.size() == 2 * P) {
+    odu.resize(P);
+    for (int p = 0; p < P; ++p) odu[p] = oldDelta[uniq[p]];
+  } else {
+    odu = oldDelta;  // size mismatch: solveBCS re-initialises the gaps
+  }
+  // G_pq = V0 int rho_i rho_j (trapezoid), symmetrised
+  vecd rhoi((size_t)P * N3);
+  for (int p = 0; p < P; ++p) {
+    const cd* ps = &psi[(size_t)uniq[p] * nr];
+    for (size_t x = 0; x < N3; ++x) rhoi[(size_t)p * N3 + x] = std::norm(ps[2 * x]) + std::norm(ps[2 * x + 1]);
+  }
+  vecd G((size_t)P * P);
+  vecd tmp(N3);
+  for (int p = 0; p < P; ++p)
+    for (int q = 0; q < P; ++q) {
+      for (size_t x = 0; x < N3; ++x) tmp[x] = rhoi[(size_t)p * N3 + x] * rhoi[(size_t)q * N3 + x];
+      G[(size_t)q * P + p] = prm.V0 * integral(g, tmp);
+    }
+  vecd Gs((size_t)P * P);
+  for (int p = 0; p < P; ++p)
+    for (int q = 0; q < P; ++q) Gs[(size_t)q * P + p] = 0.5 * (G[(size_t)q * P + p] + G[(size_t)p * P + q]);
+  BCSRes pr = solve_bcs(ep, Gs, A, prm, odu, oldLambda);
+  BCSRes R;
+  R.u2.assign(M, 0.0);
+  R.v2.assign(M, 0.0);
+  R.Delta.assign(M, 0.0);
+  R.qp.assign(M, 0.0);
+  for (int i = 0; i < M; ++i)
+    if (vis[i]) {
+      int pi = s2p[i];
+      R.u2[i] = pr.u2[pi];
+      R.v2[i] = pr.v2[pi];
+      R.Delta[i] = pr.Delta[pi];
+    }
+  for (int i = 0; i < M; ++i) R.qp[i] = eps[i] * R.v2[i];
+  R.lambda = pr.lambda;
+  R.Epair = pr.Epair;
+  return R;
+}
+
+// ------------------------------------------------------- SCF state
+struct State {
+  bool have = false;
+  vecd rho[2], tau[2], J[2], divJ[2], Jv[2], grho[2], lrho[2];
+  vecd f[2], gf[2], U[2], W[2], UCoul, UCDir;
+  vecd Uconstr;
+  BCSRes bcs[2];
+  double massCorr = MNUC;
+  double energyDiff = 1.0;
+};
+struct Con {
+  nhf_constraint c;
+  double lambda = 0.0, value = 0.0;
+  bool first = true;
+};
+
+static void op_field(const Grid& g, int kind, vecd& O) {
+  O.assign(g.N3, 0.0);
+  for (int k = 0; k < g.n; ++k)
+    for (int j = 0; j < g.n; ++j)
+      for (int i = 0; i < g.n; ++i) {
+        double x = g.x[i], y = g.x[j], z = g.x[k], v = 0;
+        switch (kind) {
+          case NHF_CON_XCM: v = x; break;
+          case NHF_CON_YCM: v = y; break;
+          case NHF_CON_ZCM: v = z; break;
+          case NHF_CON_X2MY2: v = x * x - y * y; break;
+          case NHF_CON_XY2: v = 2.0 * x * y; break;
+          case NHF_CON_OCT: {
+            double r = std::sqrt(x * x + y * y + z * z), ct = z / r;
+            v = r * r * r * std::sqrt(7.0 / (4 * PI)) * 0.5 * (5 * ct * ct * ct - 3 * ct);
+            break;
+          }
+          case NHF_CON_Q20: v = 0.25 * std::sqrt(5 / PI) * (2.0 * z * z - y * y - x * x); break;
+        }
+        O[g.idx(i, j, k)] = v;
+      }
+}
+
+struct Scf {
+  Grid g;
+  nhf_config cfg;
+  State S;
+  std::vector<Con> cons;
+  Scf(const nhf_config& c) : g(c.n, 0.5 * c.box_size), cfg(c) {
+    S.massCorr = c.com_corr ? MNUC * (double)c.A / (double)(c.A - 1) : MNUC;
+    for (int i = 0; i < c.n_con; ++i) {
+      Con k;
+      k.c = c.con[i];
+      cons.push_back(k);
+    }
+  }
+  int Nq(int q) const { return q == 0 ? cfg.A - cfg.Z : cfg.Z; }
+  const nhf_pairing& pairing(int q) const { return q == 0 ? cfg.pair_n : cfg.pair_p; }
+
+  // IterationData::solvePairingBCS (iteration_data.cpp:384-465)
+  void pairing_bcs(const vecc* psi, const vecd* eps, const int* M) {
+    BCSRes nul[2];
+    for (int q = 0; q < 2; ++q) {
+      nul[q].v2.assign(M[q], 0.0);
+      nul[q].u2.assign(M[q], 1.0);
+      for (int i = 0; i < Nq(q) && i < M[q]; ++i) {
+        nul[q].v2[i] = 1.0;
+        nul[q].u2[i] = 0.0;
+      }
+      nul[q].Delta.assign(Nq(q), 0.0);
+      nul[q].qp.resize(M[q]);
+      for (int i = 0; i < M[q]; ++i) nul[q].qp[i] = nul[q].v2[i] * eps[q][i];
+    }
+    if (cfg.pairing_type == NHF_PAIR_NONE) {
+      S.bcs[0] = nul[0];
+      S.bcs[1] = nul[1];
+      return;
+    }
+    for (int q = 0; q < 2; ++q) {
+      if (!pairing(q).active) {
+        S.bcs[q] = nul[q];
+        continue;
+      }
+      if (!S.have) {
+        vecd od(1, 0.0);
+        int N = Nq(q);
+        S.bcs[q] = bcs_iter(g, psi[q], eps[q], M[q], N, pairing(q), od, 0.5 * (eps[q][N - 1] + eps[q][N]));
+      } else {
+        S.bcs[q] = bcs_iter(g, psi[q], eps[q], M[q], Nq(q), pairing(q), S.bcs[q].Delta, S.bcs[q].lambda);
+      }
+    }
+    double sn = 0, sp = 0;
+    for (double v : S.bcs[0].v2) sn += v;
+    for (double v : S.bcs[1].v2) sp += v;
+    if (std::isnan(sn) || std::isnan(sp)) {
+      S.bcs[0] = nul[0];
+      S.bcs[1] = nul[1];
+    }
+  }
+
+  // IterationData::updateQuantities (iteration_data.cpp:852-1125), HF/BCS
+  void update(const vecc* psi, const vecd* eps, const int* M, int iter, double* tPoisson = nullptr) {
+    size_t N3 = g.N3;
+    const double* C = cfg.edf;
+    double mu = cfg.density_mix;
+    pairing_bcs(psi, eps, M);
+    Dens D[2];
+    for (int q = 0; q < 2; ++q) densities(g, psi[q], M[q], S.bcs[q].v2.data(), D[q]);
+    auto mix = [&](vecd& old, const vecd& nw) {
+      if (!S.have)
+        old = nw;
+      else
+        for (size_t i = 0; i < old.size(); ++i) old[i] = old[i] * (1.0 - mu) + nw[i] * mu;
+    };
+    for (int q = 0; q < 2; ++q) {
+      mix(S.rho[q], D[q].rho);
+      mix(S.tau[q], D[q].tau);
+      mix(S.J[q], D[q].J);
+      mix(S.divJ[q], D[q].divJ);
+      S.grho[q].assign(3 * N3, 0.0);
+      for (int ax = 0; ax < 3; ++ax) dv_real(g, S.rho[q].data(), ax, &S.grho[q][ax * N3]);
+      S.lrho[q].assign(N3, 0.0);
+      vecd t(N3);
+      for (int ax = 0; ax < 3; ++ax) {
+        d2v_real(g, S.rho[q].data(), ax, t.data());
+        for (size_t p = 0; p < N3; ++p) S.lrho[q][p] += t[p];
+      }
+      levi_civita(N3, S.J[q], S.Jv[q]);
+    }
+    vecd dJ[2];
+    for (int q = 0; q < 2; ++q) dJ[q] = cfg.spin_orbit ? S.divJ[q] : vecd(N3, 0.0);
+    double hb2m = HBAR * HBAR / (2 * S.massCorr);
+    double sig = C[NHF_EDF_SIGMA];
+    double t3 = 48 * C[NHF_EDF_C0DRR] / 3;
+    double x3 = -0.5 - 24 * C[NHF_EDF_C1DRR] / t3;
+    for (int q = 0; q < 2; ++q) {
+      S.f[q].assign(N3, 0.0);
+      S.gf[q].assign(3 * N3, 0.0);
+      S.U[q].assign(N3, 0.0);
+      for (size_t p = 0; p < N3; ++p) {
+        double rho = S.rho[0][p] + S.rho[1][p], rq = S.rho[q][p];
+        double tau = S.tau[0][p] + S.tau[1][p], tq = S.tau[q][p];
+        double lap = S.lrho[0][p] + S.lrho[1][p], lq = S.lrho[q][p];
+        double dj = dJ[0][p] + dJ[1][p], djq = dJ[q][p];
+        // EffectiveMass (effective_mass.cpp:7-34)
+        S.f[q][p] = hb2m + (C[NHF_EDF_C0RT] - C[NHF_EDF_C1RT]) * rho + 2 * C[NHF_EDF_C1RT] * rq;
+        for (int ax = 0; ax < 3; ++ax)
+          S.gf[q][ax * N3 + p] = (C[NHF_EDF_C0RT] - C[NHF_EDF_C1RT]) * (S.grho[0][ax * N3 + p] + S.grho[1][ax * N3 + p]) +
+                                 2 * C[NHF_EDF_C1RT] * S.grho[q][ax * N3 + p];
+        // Wavefunction::field (wavefunction.cpp:264-310)
+        double U = (2 * C[NHF_EDF_C0RR] - 2 * C[NHF_EDF_C1RR]) * rho + 4 * C[NHF_EDF_C1RR] * rq;
+        U += (C[NHF_EDF_C0RT] - C[NHF_EDF_C1RT]) * tau + 2 * C[NHF_EDF_C1RT] * tq;
+        U += (2 * C[NHF_EDF_C0RDR] - 2 * C[NHF_EDF_C1RDR]) * lap + 4 * C[NHF_EDF_C1RDR] * lq;
+        U += (C[NHF_EDF_C0NJ] - C[NHF_EDF_C1NJ]) * dj + 2 * C[NHF_EDF_C1NJ] * djq;
+        U += (t3 / 12.0) * std::pow(rho, sig - 1.0) *
+             ((1.0 + 0.5 * x3) * (sig + 2.0) * std::pow(rho, 2.0) -
+              (x3 + 0.5) * (sig * (rq * rq + (rho - rq) * (rho - rq)) + 2.0 * rho * rq));
+        S.U[q][p] = U;
+      }
+    }
+    // constraints (iteration_data.cpp:1031-1038): fields only for iter > 0
+    vecd cf(N3, 0.0);
+    if (iter > 0) {
+      vecd rho(N3), O;
+      for (size_t p = 0; p < N3; ++p) rho[p] = S.rho[0][p] + S.rho[1][p];
+      for (auto& k : cons) {
+        op_field(g, k.c.kind, O);
+        vecd t(N3);
+        for (size_t p = 0; p < N3; ++p) t[p] = O[p] * rho[p];
+        double Q = integral(g, t);
+        double coef = constraint_field_coef(k, Q);
+        for (size_t p = 0; p < N3; ++p) cf[p] += coef * O[p];
+      }
+    }
+    S.Uconstr = cf;
+    // Coulomb (iteration_data.cpp:1042-1060)
+    if (cfg.coulomb) {
+      auto t0 = std::chrono::steady_clock::now();
+      if (S.UCDir.size() != N3) S.UCDir.assign(N3, 0.0);
+      poisson(g, S.rho[1].data(), cfg.Z, S.UCDir.data(), 1e-13, 100000);
+      if (tPoisson) *tPoisson += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      S.UCoul.assign(N3, 0.0);
+      for (size_t p = 0; p < N3; ++p)
+        S.UCoul[p] = S.UCDir[p] - E2 * std::pow(3.0 / PI, 1.0 / 3.0) * std::pow(S.rho[1][p], 1.0 / 3.0);
+    } else {
+      S.UCoul.assign(N3, 0.0);
+    }
+    for (int q = 0; q < 2; ++q)
+      for (size_t p = 0; p < N3; ++p) S.U[q][p] += cf[p];
+    // spin-orbit form factor B_q (iteration_data.cpp:1094-1122)
+    for (int q = 0; q < 2; ++q) {
+      S.W[q].assign(3 * N3, 0.0);
+      for (size_t i = 0; i < 3 * N3; ++i) {
+        double w = 0;
+        if (cfg.spin_orbit)
+          w += -(C[NHF_EDF_C0NJ] - C[NHF_EDF_C1NJ]) * (S.grho[0][i] + S.grho[1][i]) - 2 * C[NHF_EDF_C1NJ] * S.grho[q][i];
+        if (cfg.use_j2)
+          w += 2 * (C[NHF_EDF_C0J2] - C[NHF_EDF_C1J2]) * (S.Jv[0][i] + S.Jv[1][i]) + 4 * C[NHF_EDF_C1J2] * S.Jv[q][i];
+        S.W[q][i] = w;
+      }
+    }
+    S.have = true;
+  }
+
+  // getField semantics of the constraint family (quadrupole_constraint.cpp:
+  // 22-78, xcm_constraint.cpp:14-62 and siblings)
+  double constraint_field_coef(Con& k, double Q) {
+    double C = k.c.C, tg = k.c.target;
+    bool com = k.c.kind <= NHF_CON_ZCM;
+    if (k.first) {
+      k.first = false;
+      double mu = com ? tg : tg - k.lambda / (2.0 * C);
+      return 2.0 * C * (Q - mu);
+    }
+    if (com) {
+      if (S.energyDiff < 5e-3) k.lambda += 0.2 * 2.0 * C * (Q - tg);
+    } else {
+      k.lambda += 0.3 * 2.0 * C * (Q - tg);
+    }
+    k.value = Q;
+    double mu = tg - k.lambda / (2.0 * C);
+    return 2.0 * C * (Q - mu);
+  }
+  double constraint_energy() {
+    double E = 0;
+    vecd rho(g.N3), O, t(g.N3);
+    for (size_t p = 0; p < g.N3; ++p) rho[p] = S.rho[0][p] + S.rho[1][p];
+    for (auto& k : cons) {
+      op_field(g, k.c.kind, O);
+      for (size_t p = 0; p < g.N3; ++p) t[p] = O[p] * rho[p];
+      double Q = integral(g, t);
+      E += k.c.C * (Q - k.c.target) * (Q - k.c.target) + k.lambda * (Q - k.c.target);
+    }
+    return E;
+  }
+  double constraint_error(const Con& k) const {
+    if (k.c.kind <= NHF_CON_ZCM) return 0.0;
+    if (std::abs(k.c.target) <= 1e-12) return std::abs(k.value);
+    return std::abs(k.value / k.c.target - 1.0);
+  }
+
+  // energies (iteration_data.cpp:153-271, iteration_data_integrals.cpp)
+  void energies(double* T) {
+    size_t N3 = g.N3;
+    const double* C = cfg.edf;
+    double sig = C[NHF_EDF_SIGMA];
+    vecd e(N3);
+    auto integ = [&](auto fn) {
+      for (size_t p = 0; p < N3; ++p) e[p] = fn(p);
+      return integral(g, e);
+    };
+    const vecd &rn = S.rho[0], &rp = S.rho[1];
+    std::fill(T, T + NHF_NTERMS, 0.0);
+    T[NHF_E_C0RHO] = integ([&](size_t p) {
+      double r0 = rn[p] + rp[p];
+      return (C[NHF_EDF_C0RR] + C[NHF_EDF_C0DRR] * std::pow(r0, sig)) * r0 * r0;
+    });
+    T[NHF_E_C1RHO] = integ([&](size_t p) {
+      double r0 = rn[p] + rp[p], r1 = rn[p] - rp[p];
+      return (C[NHF_EDF_C1RR] + C[NHF_EDF_C1DRR] * std::pow(r0, sig)) * r1 * r1;
+    });
+    T[NHF_E_C0LAP] = integ([&](size_t p) {
+      return C[NHF_EDF_C0RDR] * (rn[p] + rp[p]) * (S.lrho[0][p] + S.lrho[1][p]);
+    });
+    T[NHF_E_C1LAP] = integ([&](size_t p) {
+      return C[NHF_EDF_C1RDR] * (S.lrho[0][p] - S.lrho[1][p]) * (rn[p] - rp[p]);
+    });
+    T[NHF_E_C0TAU] = integ([&](size_t p) { return C[NHF_EDF_C0RT] * (S.tau[0][p] + S.tau[1][p]) * (rn[p] + rp[p]); });
+    T[NHF_E_C1TAU] = integ([&](size_t p) { return C[NHF_EDF_C1RT] * (S.tau[0][p] - S.tau[1][p]) * (rn[p] - rp[p]); });
+    T[NHF_E_C0J2] = integ([&](size_t p) {
+      double s = 0;
+      for (int a = 0; a < 3; ++a) {
+        double j = S.Jv[0][a * N3 + p] + S.Jv[1][a * N3 + p];
+        s += j * j;
+      }
+      return C[NHF_EDF_C0J2] * s;
+    });
+    T[NHF_E_C1J2] = integ([&](size_t p) {
+      double s = 0;
+      for (int a = 0; a < 3; ++a) {
+        double j = S.Jv[0][a * N3 + p] - S.Jv[1][a * N3 + p];
+        s += j * j;
+      }
+      return C[NHF_EDF_C1J2] * s;
+    });
+    T[NHF_E_C0DIVJ] = integ([&](size_t p) { return C[NHF_EDF_C0NJ] * (S.divJ[0][p] + S.divJ[1][p]) * (rn[p] + rp[p]); });
+    T[NHF_E_C1DIVJ] = integ([&](size_t p) { return C[NHF_EDF_C1NJ] * (S.divJ[0][p] - S.divJ[1][p]) * (rn[p] - rp[p]); });
+    if (cfg.coulomb) {
+      T[NHF_E_COUL_DIR] = 0.5 * integ([&](size_t p) { return rp[p] * S.UCDir[p]; });
+      T[NHF_E_COUL_EX] = integ([&](size_t p) {
+        return -3.0 * E2 / 4.0 * std::pow(3.0 / PI, 1.0 / 3.0) * std::pow(rp[p], 4.0 / 3.0);
+      });
+    }
+    T[NHF_E_KIN] = integ([&](size_t p) {
+      return 0.5 * HBAR * HBAR * S.tau[0][p] / S.massCorr + 0.5 * HBAR * HBAR * S.tau[1][p] / S.massCorr;
+    });
+    auto epair = [&](int q) { return (cfg.pairing_type == NHF_PAIR_BCS && pairing(q).active) ? S.bcs[q].Epair : 0.0; };
+    T[NHF_E_PAIR_N] = epair(0);
+    T[NHF_E_PAIR_P] = epair(1);
+    T[NHF_E_REAR] = integ([&](size_t p) {
+      double r0 = rn[p] + rp[p], r1 = rn[p] - rp[p];
+      return -0.5 * sig * (C[NHF_EDF_C0DRR] * std::pow(r0, sig + 2) + C[NHF_EDF_C1DRR] * std::pow(r0, sig) * r1 * r1);
+    });
+    T[NHF_E_CONSTR] = constraint_energy();
+    double Eint = 0;
+    for (int t = NHF_E_C0RHO; t <= NHF_E_PAIR_P; ++t) Eint += T[t];
+    T[NHF_E_INT] = Eint;
+    double qp = 0;
+    for (int q = 0; q < 2; ++q)
+      for (double v : S.bcs[q].qp) qp += v;
+    T[NHF_E_HF] = T[NHF_E_CONSTR] + 0.5 * (qp + T[NHF_E_KIN]) + T[NHF_E_REAR] + T[NHF_E_COUL_EX] / 3.0 +
+                  T[NHF_E_PAIR_N] + T[NHF_E_PAIR_P];
+  }
+
+  void observables(nhf_result* R) {
+    size_t N3 = g.N3;
+    vecd t(N3);
+    double r2[2];
+    for (int q = 0; q < 2; ++q) {
+      for (int k = 0; k < g.n; ++k)
+        for (int j = 0; j < g.n; ++j)
+          for (int i = 0; i < g.n; ++i) {
+            size_t p = g.idx(i, j, k);
+            t[p] = (g.x[i] * g.x[i] + g.x[j] * g.x[j] + g.x[k] * g.x[k]) * S.rho[q][p];
+          }
+      r2[q] = integral(g, t) / integral(g, S.rho[q]);
+    }
+    R->r2_n = r2[0];
+    R->r2_p = r2[1];
+    // quadrupoleDeformation (iteration_data_integrals.cpp:51-67)
+    vecd t2(N3);
+    for (int k = 0; k < g.n; ++k)
+      for (int j = 0; j < g.n; ++j)
+        for (int i = 0; i < g.n; ++i) {
+          size_t p = g.idx(i, j, k);
+          double x = g.x[i], y = g.x[j], z = g.x[k], r = std::sqrt(x * x + y * y + z * z);
+          double rho = S.rho[0][p] + S.rho[1][p];
+          t[p] = r * r * Y20c(x, y, z) * rho;
+          t2[p] = r * r * Y22c(x, y, z) * rho;
+        }
+    double a20 = integral(g, t), a22 = integral(g, t2);
+    double A = cfg.A, R0 = 1.2 * std::pow(A, 1.0 / 3.0);
+    R->beta2 = 4 * PI * a20 / (3 * A * R0 * R0);
+    R->gamma = std::atan2(std::sqrt(2.0) * a22, a20);
+    R->lambda_n = S.bcs[0].lambda;
+    R->lambda_p = S.bcs[1].lambda;
+  }
+};
+
+// Wavefunction::normalize (wavefunction.cpp:362-370)
+static void normalize(const Grid& g, vecc& psi, int m) {
+  size_t nr = 2 * g.N3;
+  vecd d(g.N3);
+  for (int c = 0; c < m; ++c) {
+    cd* ps = &psi[(size_t)c * nr];
+    for (size_t p = 0; p < g.N3; ++p) d[p] = std::norm(ps[2 * p]) + std::norm(ps[2 * p + 1]);
+    double s = std::sqrt(integral(g, d));
+    for (size_t i = 0; i < nr; ++i) ps[i] /= s;
+  }
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ======================================================================
+// C ABI for the tests (ctypes).  Complex arrays are interleaved doubles.
+extern "C" {
+
+void orc_edf_from_skyrme(const double* skyrme, double* edf) { edf_from_skyrme(skyrme, edf); }
+
+void orc_dv_real(int n, double a, const double* f, int axis, double* out) {
+  Grid g(n, a);
+  dv_real(g, f, axis, out);
+}
+void orc_d2v_real(int n, double a, const double* f, int axis, double* out) {
+  Grid g(n, a);
+  d2v_real(g, f, axis, out);
+}
+void orc_dv_spinor(int n, double a, const double* psi, int axis, double* out) {
+  Grid g(n, a);
+  dv_spinor(g, (const cd*)psi, axis, (cd*)out);
+}
+double orc_integral(int n, double a, const double* f) {
+  Grid g(n, a);
+  return integral(g, f);
+}
+
+static Fields mk_fields(const Grid& g, const double* f, const double* gf, const double* U, const double* W) {
+  Fields F;
+  F.f.assign(f, f + g.N3);
+  F.gf.assign(gf, gf + 3 * g.N3);
+  F.U.assign(U, U + g.N3);
+  F.W.assign(W, W + 3 * g.N3);
+  return F;
+}
+
+void orc_hpsi(int n, double a, const double* f, const double* gf, const double* U, const double* W, const double* psi,
+              int m, double* out) {
+  Grid g(n, a);
+  Fields F = mk_fields(g, f, gf, U, W);
+  size_t nr = 2 * g.N3;
+  for (int c = 0; c < m; ++c) hpsi_col(g, F, (const cd*)psi + c * nr, (cd*)out + c * nr);
+}
+
+long orc_build_csr(int n, double a, const double* f, const double* gf, const double* U, const double* W, int so,
+                   long* rowptr, int* col, double* val, long cap) {
+  Grid g(n, a);
+  Fields F = mk_fields(g, f, gf, U, W);
+  return build_csr(g, F, so != 0, rowptr, col, (cd*)val, cap);
+}
+
+int orc_eigh(int k, const double* A, double* w, double* V) { return eigh_lower(k, (const cd*)A, w, (cd*)V) ? 0 : -1; }
+
+int orc_cg(int n, double a, const double* f, const double* gf, const double* U, const double* W, double shift,
+           const double* b, double* x, int maxIters, double tol) {
+  Grid g(n, a);
+  Fields F = mk_fields(g, f, gf, U, W);
+  Op A{&g, &F, shift};
+  return eigen_cg(A, (const cd*)b, (cd*)x, maxIters, tol);
+}
+
+int orc_gcg(int n, double a, const double* f, const double* gf, const double* U, const double* W, const double* X,
+            int nev, const double* P, int pc, const nhf_gcg_params* prm, double* Xout, double* lam, int* cg_iters) {
+  Grid g(n, a);
+  Fields F = mk_fields(g, f, gf, U, W);
+  size_t nr = 2 * g.N3;
+  vecc Xi((const cd*)X, (const cd*)X + (size_t)nev * nr);
+  vecc Pi;
+  if (pc > 0) Pi.assign((const cd*)P, (const cd*)P + (size_t)pc * nr);
+  vecc Xo;
+  vecd lo;
+  GcgStats st;
+  if (!gcg(g, F, Xi, nev, Pi, pc, *prm, Xo, lo, &st)) return -1;
+  std::copy(Xo.begin(), Xo.end(), (cd*)Xout);
+  std::copy(lo.begin(), lo.end(), lam);
+  if (cg_iters) *cg_iters = st.cg_iters_total;
+  return 0;
+}
+
+void orc_densities(int n, double a, const double* psi, int m, const double* occ, double* rho, double* tau, double* J,
+                   double* divJ) {
+  Grid g(n, a);
+  vecc P((const cd*)psi, (const cd*)psi + (size_t)m * 2 * g.N3);
+  Dens D;
+  densities(g, P, m, occ, D);
+  std::copy(D.rho.begin(), D.rho.end(), rho);
+  std::copy(D.tau.begin(), D.tau.end(), tau);
+  std::copy(D.J.begin(), D.J.end(), J);
+  std::copy(D.divJ.begin(), D.divJ.end(), divJ);
+}
+
+void orc_coulomb_rhs(int n, double a, const double* rhoP, int Z, double* rhs) {
+  Grid g(n, a);
+  vecd r;
+  coulomb_rhs(g, rhoP, Z, r);
+  std::copy(r.begin(), r.end(), rhs);
+}
+void orc_lap4(int n, double a, const double* u, double* out) {
+  Grid g(n, a);
+  lap4(g, u, out);
+}
+int orc_poisson(int n, double a, const double* rhoP, int Z, double* U, double tol, int maxit) {
+  Grid g(n, a);
+  return poisson(g, rhoP, Z, U, tol, maxit);
+}
+
+// BCS on given pair energies and G (P x P, column-major): returns v2, Delta
+int orc_solve_bcs(int P, const double* eps, const double* G, int A, double V0, double window, int window_both,
+                  double lambda0, double* v2, double* Delta, double* lambda, double* Epair) {
+  vecd e(eps, eps + P), Gv(G, G + (size_t)P * P), od(1, 0.0);
+  nhf_pairing prm{V0, window, window_both, 1};
+  BCSRes R = solve_bcs(e, Gv, A, prm, od, lambda0);
+  std::copy(R.v2.begin(), R.v2.end(), v2);
+  std::copy(R.Delta.begin(), R.Delta.end(), Delta);
+  *lambda = R.lambda;
+  *Epair = R.Epair;
+  return 0;
+}
+
+void orc_time_reverse(int n, const double* psi, int m, double* out) {
+  size_t N3 = (size_t)n * n * n;
+  for (int c = 0; c < m; ++c) time_reverse(N3, (const cd*)psi + (size_t)c * 2 * N3, (cd*)out + (size_t)c * 2 * N3);
+}
+
+// HO guess (guess.cpp:10-248, useSpinOrbit = true): Euclidean-normalised.
+int orc_ho_guess(int n, double box, int nev, double b, double* out) {
+  Grid g(n, 0.5 * box);
+  size_t nr = 2 * g.N3;
+  cd* G = (cd*)out;
+  std::fill(G, G + (size_t)nev * nr, cd(0.0, 0.0));
+  auto laguerre = [](int nn, double al, double x) {
+    if (nn == 0) return 1.0;
+    if (nn == 1) return 1.0 + al - x;
+    double a2 = 1.0, a1 = 1.0 + al - x, an = 0.0;
+    for (int k = 2; k <= nn; ++k) {
+      an = ((2.0 * k - 1.0 + al - x) * a1 - (k - 1.0 + al) * a2) / k;
+      a2 = a1;
+      a1 = an;
+    }
+    return an;
+  };
+  auto radial = [&](int nrr, int l, double r) {
+    double rh = r / b;
+    double nm = std::sqrt(2.0 * std::tgamma(nrr + 1.0) / (b * b * b * std::tgamma(nrr + l + 1.5)));
+    return nm * std::pow(rh, l) * laguerre(nrr, l + 0.5, rh * rh) * std::exp(-0.5 * rh * rh);
+  };
+  auto fact = [](int k) {
+    double r = 1;
+    for (int i = 2; i <= k; ++i) r *= i;
+    return r;
+  };
+  auto legendre = [](int l, int m, double x) {
+    double pmm = 1.0;
+    if (m > 0) {
+      double fct = 1.0;
+      for (int i = 1; i <= 2 * m - 1; i += 2) fct *= i;
+      pmm = fct * std::pow(1.0 - x * x, m / 2.0);
+    }
+    if (l == m) return pmm;
+    double pmmp1 = x * (2 * m + 1) * pmm;
+    if (l == m + 1) return pmmp1;
+    double pll = 0, p1 = pmmp1, p2 = pmm;
+    for (int ll = m + 2; ll <= l; ++ll) {
+      pll = ((2 * ll - 1) * x * p1 - (ll + m - 1) * p2) / (ll - m);
+      p2 = p1;
+      p1 = pll;
+    }
+    return pll;
+  };
+  auto Y = [&](int l, int m, double th, double ph) {
+    int mm = std::abs(m);
+    double nm = std::sqrt(((2.0 * l + 1.0) / (4.0 * PI)) * (fact(l - mm) / fact(l + mm)));
+    double lg = legendre(l, mm, std::cos(th));
+    cd res = std::pow(-1.0, mm) * nm * lg * std::polar(1.0, mm * ph);
+    if (m < 0) return std::pow(-1.0, mm) * std::conj(res);
+    return res;
+  };
+  std::vector<std::tuple<int, int, int, int>> orb;
+  for (int N = 0; (int)orb.size() < nev; ++N)
+    for (int nrr = 0; nrr <= N / 2 && (int)orb.size() < nev; ++nrr) {
+      int l = N - 2 * nrr;
+      int jp = 2 * l + 1;
+      for (int ma = 1; ma <= jp && (int)orb.size() < nev; ma += 2) {
+        orb.emplace_back(nrr, l, jp, ma);
+        if ((int)orb.size() < nev) orb.emplace_back(nrr, l, jp, -ma);
+      }
+      if (l > 0) {
+        int jm = 2 * l - 1;
+        for (int ma = 1; ma <= jm && (int)orb.size() < nev; ma += 2) {
+          orb.emplace_back(nrr, l, jm, ma);
+          if ((int)orb.size() < nev) orb.emplace_back(nrr, l, jm, -ma);
+        }
+      }
+    }
+  for (int si = 0; si < nev; ++si) {
+    int nrr, l, j2, mj2;
+    std::tie(nrr, l, j2, mj2) = orb[si];
+    double mj = mj2 * 0.5, den = 2.0 * l + 1.0, Cu, Cd;
+    if (j2 == 2 * l + 1) {
+      Cu = std::sqrt((l + mj + 0.5) / den);
+      Cd = std::sqrt((l - mj + 0.5) / den);
+    } else {
+      Cu = std::sqrt((l - mj + 0.5) / den);
+      Cd = -std::sqrt((l + mj - 0.5) / den);
+    }
+    int mlu = (mj2 - 1) / 2, mld = (mj2 + 1) / 2;
+    bool hu = std::abs(mlu) <= l, hd = std::abs(mld) <= l;
+    cd* col = G + (size_t)si * nr;
+    for (int i = 0; i < n; ++i)
+      for (int jg = 0; jg < n; ++jg)
+        for (int k = 0; k < n; ++k) {
+          double x = g.x[i], y = g.x[jg], z = g.x[k];
+          double r = std::sqrt(x * x + y * y + z * z);
+          double th = (r > 1e-12) ? std::acos(z / r) : 0.0, ph = std::atan2(y, x);
+          double R = radial(nrr, l, r);
+          cd up = hu ? (Cu * R) * Y(l, mlu, th, ph) : cd(0, 0);
+          cd dn = hd ? (Cd * R) * Y(l, mld, th, ph) : cd(0, 0);
+          size_t p = g.idx(i, jg, k);
+          col[2 * p] = up;
+          col[2 * p + 1] = dn;
+        }
+    double s = std::sqrt(nrm2sq(col, nr));
+    for (size_t r = 0; r < nr; ++r) col[r] /= s;
+  }
+  return 0;
+}
+
+// Full SCF loop (main.cpp:199-342) in FP64.  psi_n/psi_p in/out
+// (2n^3 x M), trace rows [E_int, E_HF, max|deps|, seconds].
+int orc_scf_run(const nhf_config* cfg, double* psi_n, double* psi_p, double* eps_n, double* eps_p, nhf_result* res,
+                double* trace, int trace_cap) {
+  Scf scf(*cfg);
+  Grid& g = scf.g;
+  size_t nr = 2 * g.N3;
+  int M[2] = {cfg->M_n, cfg->M_p};
+  vecc psi[2];
+  psi[0].assign((const cd*)psi_n, (const cd*)psi_n + (size_t)M[0] * nr);
+  psi[1].assign((const cd*)psi_p, (const cd*)psi_p + (size_t)M[1] * nr);
+  vecd eps[2];
+  eps[0].assign(M[0], 0.0);
+  eps[1].assign(M[1], 0.0);
+  // the reference enters the loop with the start-up eigenvalues; for a
+  // guess without them the occupation-weighted quantities of iteration 0
+  // only need eps for qpEnergies and the first BCS lambda.
+  for (int q = 0; q < 2; ++q)
+    for (int i = 0; i < M[q]; ++i) eps[q][i] = (q == 0 ? eps_n : eps_p)[i];
+  vecc conj[2];
+  int pc[2] = {0, 0};
+  double Eint_prev = 0.0, Ehf_prev = 0.0;
+  int N = cfg->A - cfg->Z, Zp = cfg->Z;
+  auto t0 = std::chrono::steady_clock::now();
+  int it;
+  bool conv = false;
+  double T[NHF_NTERMS];
+  double tstop = -1.0;
+  for (it = 0; it < cfg->max_iter; ++it) {
+    scf.update(psi, eps, M, it);
+    vecc X[2];
+    vecd L[2];
+    for (int q = 0; q < 2; ++q) {
+      if (q == 1 && !(cfg->coulomb || N != Zp)) {
+        X[1].assign(X[0].begin(), X[0].begin() + (size_t)M[1] * nr);
+        L[1].assign(L[0].begin(), L[0].begin() + M[1]);
+        continue;
+      }
+      Fields F;
+      F.f = scf.S.f[q];
+      F.gf = scf.S.gf[q];
+      F.U = scf.S.U[q];
+      if (q == 1)
+        for (size_t p = 0; p < g.N3; ++p) F.U[p] += scf.S.UCoul[p];
+      F.W = scf.S.W[q];
+      if (!gcg(g, F, psi[q], M[q], conj[q], pc[q], cfg->gcg, X[q], L[q])) return NHF_ESOLVER;
+    }
+    double mx = 0;
+    for (int q = 0; q < 2; ++q)
+      for (int i = 0; i < M[q]; ++i) mx = std::max(mx, std::abs(L[q][i] - eps[q][i]));
+    for (int q = 0; q < 2; ++q) normalize(g, X[q], M[q]);
+    if (cfg->pairing_type != NHF_PAIR_NONE)
+      for (int q = 0; q < 2; ++q) tr_order(g.N3, X[q], M[q]);
+    if (it > 0) {
+      for (int q = 0; q < 2; ++q) {
+        int m = M[q];
+        vecc tmp((size_t)m * m);
+        for (int c = 0; c < m; ++c)
+          for (int r = 0; r < m; ++r) tmp[(size_t)c * m + r] = dotc(&psi[q][(size_t)r * nr], &X[q][(size_t)c * nr], nr);
+        vecc Y;
+        rotate(psi[q], nr, m, tmp.data(), m, Y);
+        conj[q].assign((size_t)m * nr, 0.0);
+        for (int c = 0; c < m; ++c) {
+          double s = 0;
+          for (size_t i = 0; i < nr; ++i) {
+            cd v = X[q][(size_t)c * nr + i] - Y[(size_t)c * nr + i];
+            conj[q][(size_t)c * nr + i] = v;
+            s += std::norm(v);
+          }
+          s = std::sqrt(s);
+          if (s > 0)
+            for (size_t i = 0; i < nr; ++i) conj[q][(size_t)c * nr + i] /= s;
+        }
+        pc[q] = m;
+      }
+    }
+    for (int q = 0; q < 2; ++q) {
+      psi[q].swap(X[q]);
+      eps[q] = L[q];
+    }
+    scf.energies(T);
+    double Eint = T[NHF_E_INT], Ehf = T[NHF_E_HF];
+    bool cc = true;
+    for (auto& k : scf.cons) {
+      double e = scf.constraint_error(k);
+      if (e != 0.0) cc = cc && (e < cfg->constraints_tol);
+    }
+    double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (trace && it < trace_cap) {
+      trace[4 * it + 0] = Eint;
+      trace[4 * it + 1] = Ehf;
+      trace[4 * it + 2] = mx;
+      trace[4 * it + 3] = el;
+    }
+    if (tstop < 0 && std::isfinite(cfg->stop_energy) && std::abs(Eint - cfg->stop_energy) <= cfg->stop_window)
+      tstop = el;
+    if (std::abs(Eint - Eint_prev) < cfg->energy_tol && std::abs(Ehf - Ehf_prev) < cfg->energy_tol && cc) {
+      conv = true;
+      ++it;
+      break;
+    }
+    scf.S.energyDiff = std::abs(Ehf - Ehf_prev);
+    Eint_prev = Eint;
+    Ehf_prev = Ehf;
+  }
+  res->iterations = it;
+  res->converged = conv ? 1 : 0;
+  std::copy(T, T + NHF_NTERMS, res->terms);
+  scf.observables(res);
+  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  res->seconds_to_stop = tstop;
+  std::copy(psi[0].begin(), psi[0].end(), (cd*)psi_n);
+  std::copy(psi[1].begin(), psi[1].end(), (cd*)psi_p);
+  std::copy(eps[0].begin(), eps[0].end(), eps_n);
+  std::copy(eps[1].begin(), eps[1].end(), eps_p);
+  return 0;
+}
+
+// One iteration's fields from given states (for field-level parity tests):
+// runs updateQuantities once from scratch (iteration 0: no mixing) and
+// returns f, gradf, U (with Coulomb for q = 1), W of both species and the
+// energy terms.
+int orc_fields_once(const nhf_config* cfg, const double* psi_n, const double* psi_p, const double* eps_n,
+                    const double* eps_p, double* f, double* gf, double* U, double* W, double* terms) {
+  Scf scf(*cfg);
+  Grid& g = scf.g;
+  size_t nr = 2 * g.N3, N3 = g.N3;
+  int M[2] = {cfg->M_n, cfg->M_p};
+  vecc psi[2];
+  psi[0].assign((const cd*)psi_n, (const cd*)psi_n + (size_t)M[0] * nr);
+  psi[1].assign((const cd*)psi_p, (const cd*)psi_p + (size_t)M[1] * nr);
+  vecd eps[2];
+  eps[0].assign(eps_n, eps_n + M[0]);
+  eps[1].assign(eps_p, eps_p + M[1]);
+  scf.update(psi, eps, M, 0);
+  for (int q = 0; q < 2; ++q) {
+    std::copy(scf.S.f[q].begin(), scf.S.f[q].end(), f + q * N3);
+    std::copy(scf.S.gf[q].begin(), scf.S.gf[q].end(), gf + q * 3 * N3);
+    for (size_t p = 0; p < N3; ++p) U[q * N3 + p] = scf.S.U[q][p] + (q == 1 ? scf.S.UCoul[p] : 0.0);
+    std::copy(scf.S.W[q].begin(), scf.S.W[q].end(), W + q * 3 * N3);
+  }
+  scf.energies(terms);
+  return 0;
+}
+
+}  // extern "C"
