@@ -224,6 +224,18 @@ int nhf_bench_hpsi(void *ctx, int pid, int q, int m, int prec, int reps, double 
  * useSpinOrbit = true).  Host-side; out: 2n^3 x nev complex. */
 int nhf_ho_guess(int n, double box_size, int nev, double b, double *out);
 
+/* Host FP64 eigensolver of the Rayleigh-Ritz step: eigenpairs of the
+ * Hermitian matrix defined by the LOWER triangle of A (k x k complex,
+ * column-major), ascending -- the semantics of Eigen's
+ * SelfAdjointEigenSolver as used at solver.cpp:768.  w: k, V: k x k. */
+int nhf_eigh_lower(int k, const double *A, double *w, double *V);
+
+/* Host Cholesky-QR coefficients with the column-dropping rule of the
+ * modified Gram-Schmidt + keep filter (solver.cpp:389-416, 756-764) on a
+ * Gram matrix G (k x k, lower triangle used): returns ke (kept columns),
+ * keep[0..ke) their indices, Cinv (k x ke complex) with V*Cinv orthonormal. */
+int nhf_cholqr_coeffs(int k, const double *G, double drop_rel, int *keep, double *Cinv);
+
 /* Skyrme (t,x,W0,sigma) -> EDF coupling constants (EDF.cpp:76-98).
  * skyrme: t0 t1 t2 t3 x0 x1 x2 x3 W0 W0' te to sigma. */
 void nhf_edf_from_skyrme(const double skyrme[13], double edf[NHF_EDF_N]);

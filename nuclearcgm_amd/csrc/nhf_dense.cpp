@@ -188,3 +188,20 @@ int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, s
 }
 
 }  // namespace nhf
+
+// ---------------------------------------------------------------- C ABI
+extern "C" int nhf_eigh_lower(int k, const double* A, double* w, double* V) {
+  if (k < 1 || !A || !w || !V) return NHF_EINVAL;
+  return nhf::eigh_lower(k, reinterpret_cast<const nhf::cd*>(A), w, reinterpret_cast<nhf::cd*>(V)) ? NHF_OK
+                                                                                                   : NHF_ESOLVER;
+}
+
+extern "C" int nhf_cholqr_coeffs(int k, const double* G, double drop_rel, int* keep, double* Cinv) {
+  if (k < 1 || !G || !keep || !Cinv) return NHF_EINVAL;
+  std::vector<int> kp;
+  std::vector<nhf::cd> Ci;
+  int ke = nhf::cholqr_coeffs(k, reinterpret_cast<const nhf::cd*>(G), drop_rel, kp, Ci);
+  for (int i = 0; i < ke; ++i) keep[i] = kp[i];
+  std::copy(Ci.begin(), Ci.end(), reinterpret_cast<nhf::cd*>(Cinv));
+  return ke;
+}

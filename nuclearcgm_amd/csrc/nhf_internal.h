@@ -29,11 +29,13 @@ constexpr int NFLD = 8;
 constexpr int CG_NS = 4;
 
 // ------------------------------------------------------------------ kernels
-// All launches go to `st`.  Sizes: N3 = n^3 lattice points, a column of
+// All launches go to `st`.  launch_hpsi: out = (H + shift) in; when dot is
+// non-null also accumulates in^H out of column c into dot[c * dot_stride + 0..1]
+// and skips columns with done[c] != 0.  Sizes: N3 = n^3 lattice points, a column of
 // spinors has N3 Sp<R> entries; `ld` is the column stride in Sp<R> units.
 template <typename R>
 void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int ncols, const R* fld, int n, double h,
-                 double shift, double* dot, const int* done);
+                 double shift, double* dot, int dot_stride, const int* done);
 
 // block Jacobi-PCG kernels (Eigen ConjugateGradient restated per column)
 template <typename R>

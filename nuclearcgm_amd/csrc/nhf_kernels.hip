@@ -69,7 +69,8 @@ __device__ __forceinline__ double trap_w(int i, int j, int k, int n) {
 template <typename R, bool DOT>
 __global__ __launch_bounds__(BS) void k_hpsi(const Sp<R>* __restrict__ in, Sp<R>* __restrict__ out, size_t ld,
                                              const R* __restrict__ fld, int n, size_t N3, R c1, R c2, R shift,
-                                             double* __restrict__ dot, const int* __restrict__ done) {
+                                             double* __restrict__ dot, int dot_stride,
+                                             const int* __restrict__ done) {
   const int c = blockIdx.y;
   if (done && done[c]) return;  // uniform over the workgroup
   const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
@@ -130,25 +131,26 @@ __global__ __launch_bounds__(BS) void k_hpsi(const Sp<R>* __restrict__ in, Sp<R>
       acc[1] = (double)a0.ur * o.ui - (double)a0.ui * o.ur + (double)a0.dr * o.di - (double)a0.di * o.dr;
     }
   }
-  if (DOT) block_atomic_add<2>(acc, dot + (size_t)2 * c);
+  if (DOT) block_atomic_add<2>(acc, dot + (size_t)dot_stride * c);
 }
 
 template <typename R>
 void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int ncols, const R* fld, int n, double h,
-                 double shift, double* dot, const int* done) {
+                 double shift, double* dot, int dot_stride, const int* done) {
   size_t N3 = (size_t)n * n * n;
   dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
   R c1 = (R)(1.0 / (12.0 * h)), c2 = (R)(1.0 / (12.0 * h * h));
   if (dot)
-    hipLaunchKernelGGL((k_hpsi<R, true>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot, done);
+    hipLaunchKernelGGL((k_hpsi<R, true>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot,
+                       dot_stride, done);
   else
     hipLaunchKernelGGL((k_hpsi<R, false>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot,
-                       done);
+                       dot_stride, done);
 }
 template void launch_hpsi<double>(hipStream_t, const Sp<double>*, Sp<double>*, size_t, int, const double*, int, double,
-                                  double, double*, const int*);
+                                  double, double*, int, const int*);
 template void launch_hpsi<float>(hipStream_t, const Sp<float>*, Sp<float>*, size_t, int, const float*, int, double,
-                                 double, double*, const int*);
+                                 double, double*, int, const int*);
 
 // ======================================================== block Jacobi-PCG
 // Per column c the iteration of Eigen's ConjugateGradient with the diagonal
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(BS) void k_cg_update(Sp<R>* __restrict__ W, Sp<R>* 
   const double an = sprev[(size_t)c * CG_NS + 3];
   const double pr = scur[(size_t)c * CG_NS + 0], pi = scur[(size_t)c * CG_NS + 1];
   const double den = pr * pr + pi * pi;
-  const R ar = (R)(an * pr / den), ai = (R)(-an * pi / den);
+  const R ar = den > 0.0 ? (R)(an * pr / den) : R(0), ai = den > 0.0 ? (R)(-an * pi / den) : R(0);
   double acc[2] = {0.0, 0.0};
   if (p < N3) {
     const size_t o = (size_t)c * ld + p;

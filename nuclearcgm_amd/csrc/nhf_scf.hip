@@ -278,7 +278,7 @@ static int gcg_dev(Problem& P, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const
   if (ke < M) return NHF_ESOLVER;
   CHK(rotate_dev<R>(P, psi_in, M, C1, M, V1, nullptr, nullptr, false));
   // 2. Rayleigh-Ritz on X with shift 0
-  launch_hpsi<R>(st, V1, AV, ld, M, fld, P.n, P.h, 0.0, nullptr, nullptr);
+  launch_hpsi<R>(st, V1, AV, ld, M, fld, P.n, P.h, 0.0, nullptr, 0, nullptr);
   CHK(gram_host<R>(P, V1, M, AV, M, Pm));
   std::vector<double> w(M);
   std::vector<cd> Cx((size_t)M * M);
@@ -296,7 +296,7 @@ static int gcg_dev(Problem& P, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const
     Sp<R>* T = V1 + (size_t)2 * M * ld;
     // 3. block Jacobi-PCG
     launch_invdiag<R>(st, fld, N3, P.h, shift, invd);
-    launch_hpsi<R>(st, X, T, ld, M, fld, P.n, P.h, shift, nullptr, nullptr);
+    launch_hpsi<R>(st, X, T, ld, M, fld, P.n, P.h, shift, nullptr, 0, nullptr);
     std::vector<double> ls(M);
     for (int c = 0; c < M; ++c) ls[c] = Lam[c] + shift;
     HIPCHK(hipMemcpyAsync(P.lam_shift, ls.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
@@ -310,7 +310,7 @@ static int gcg_dev(Problem& P, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const
     for (int it = 0; it < steps; ++it) {
       double* sprev = P.cg_scal + (size_t)it * M * CG_NS;
       double* scur = P.cg_scal + (size_t)(it + 1) * M * CG_NS;
-      launch_hpsi<R>(st, Pd, T, ld, M, fld, P.n, P.h, shift, scur, P.done);
+      launch_hpsi<R>(st, Pd, T, ld, M, fld, P.n, P.h, shift, scur, CG_NS, P.done);
       launch_cg_update<R>(st, W, Rr, Pd, T, invd, N3, ld, M, sprev, scur, P.done);
       launch_cg_dir<R>(st, Rr, Pd, invd, N3, ld, M, sprev, scur, P.thr, P.done, P.its);
       if ((it % 8) == 7 && it + 1 < steps) {
@@ -343,13 +343,15 @@ static int gcg_dev(Problem& P, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const
     // 5. Cholesky-QR twice with column dropping
     CHK(gram_host<R>(P, V, k, V, k, G));
     int k1 = cholqr_coeffs(k, G.data(), 1e-14, keep, C1);
+    if (k1 < 1) return NHF_ESOLVER;
     CHK(rotate_dev<R>(P, V, k, C1, k1, V1, nullptr, nullptr, false));
     CHK(gram_host<R>(P, V1, k1, V1, k1, G));
     // C2: k1 x k2 (columns that lost rank in the first pass drop out here)
     int k2 = cholqr_coeffs(k1, G.data(), 1e-14, keep, C2);
+    if (k2 < 1) return NHF_ESOLVER;
     if (info) info->basis = k2;
     // 6. Rayleigh-Ritz: Pm = C2^H (V1^H (H + sigma) V1) C2
-    launch_hpsi<R>(st, V1, AV, ld, k1, fld, P.n, P.h, shift, nullptr, nullptr);
+    launch_hpsi<R>(st, V1, AV, ld, k1, fld, P.n, P.h, shift, nullptr, 0, nullptr);
     std::vector<cd> P1;
     CHK(gram_host<R>(P, V1, k1, AV, k1, P1));
     Pm.assign((size_t)k2 * k2, 0.0);
@@ -979,10 +981,10 @@ int nhf_hpsi(void* ctx, int pid, int q, const double* psi, double* hpsi, int m, 
   if (rc == NHF_OK) {
     hipError_t e = hipMemcpyAsync(in, psi, sizeof(Sp<double>) * cnt, hipMemcpyHostToDevice, P->st);
     if (prec == NHF_PREC_FP64) {
-      launch_hpsi<double>(P->st, in, out, P->N3, m, P->fld64[q], P->n, P->h, 0.0, nullptr, nullptr);
+      launch_hpsi<double>(P->st, in, out, P->N3, m, P->fld64[q], P->n, P->h, 0.0, nullptr, 0, nullptr);
     } else {
       launch_convert<double, float>(P->st, in, in32, cnt);
-      launch_hpsi<float>(P->st, in32, out32, P->N3, m, P->fld32[q], P->n, P->h, 0.0, nullptr, nullptr);
+      launch_hpsi<float>(P->st, in32, out32, P->N3, m, P->fld32[q], P->n, P->h, 0.0, nullptr, 0, nullptr);
       launch_convert<float, double>(P->st, out32, out, cnt);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(hpsi, out, sizeof(Sp<double>) * cnt, hipMemcpyDeviceToHost, P->st);
@@ -1113,20 +1115,20 @@ int nhf_bench_hpsi(void* ctx, int pid, int q, int m, int prec, int reps, double*
   if (prec == NHF_PREC_FP64) {
     HIPCHK(hipMemsetAsync(P->V, 0, sizeof(Sp<double>) * cnt, P->st));
     launch_hpsi<double>(P->st, (Sp<double>*)P->V, (Sp<double>*)P->V1, P->N3, m, P->fld64[q], P->n, P->h, 0.0,
-                        nullptr, nullptr);
+                        nullptr, 0, nullptr);
   } else {
     HIPCHK(hipMemsetAsync(P->V, 0, sizeof(Sp<float>) * cnt, P->st));
     launch_hpsi<float>(P->st, (Sp<float>*)P->V, (Sp<float>*)P->V1, P->N3, m, P->fld32[q], P->n, P->h, 0.0, nullptr,
-                       nullptr);
+                       0, nullptr);
   }
   HIPCHK(hipEventRecord(e0, P->st));
   for (int r = 0; r < reps; ++r) {
     if (prec == NHF_PREC_FP64)
       launch_hpsi<double>(P->st, (Sp<double>*)P->V, (Sp<double>*)P->V1, P->N3, m, P->fld64[q], P->n, P->h, 0.0,
-                          nullptr, nullptr);
+                          nullptr, 0, nullptr);
     else
       launch_hpsi<float>(P->st, (Sp<float>*)P->V, (Sp<float>*)P->V1, P->N3, m, P->fld32[q], P->n, P->h, 0.0, nullptr,
-                         nullptr);
+                         0, nullptr);
   }
   HIPCHK(hipEventRecord(e1, P->st));
   HIPCHK(hipEventSynchronize(e1));

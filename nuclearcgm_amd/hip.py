@@ -45,6 +45,8 @@ def lib():
         L.nhf_bench_hpsi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _D]
         L.nhf_ho_guess.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]
+        L.nhf_eigh_lower.argtypes = [ctypes.c_int, _D, _D, _D]
+        L.nhf_cholqr_coeffs.argtypes = [ctypes.c_int, _D, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _D]
         L.nhf_edf_from_skyrme.argtypes = [_D, _D]
         L.nhf_edf_from_skyrme.restype = None
         _lib = L
@@ -75,6 +77,27 @@ def edf_from_skyrme(sk):
     out = np.zeros(13)
     lib().nhf_edf_from_skyrme(_dp(sk), _dp(out))
     return out
+
+
+def eigh_lower(A):
+    """Host FP64 Rayleigh-Ritz eigensolver of the library (lower triangle)."""
+    A = np.asfortranarray(A, dtype=np.complex128)
+    k = A.shape[0]
+    w = np.zeros(k)
+    V = np.zeros((k, k), dtype=np.complex128, order="F")
+    _chk(lib().nhf_eigh_lower(k, _dp(A), _dp(w), _dp(V)))
+    return w, V
+
+
+def cholqr_coeffs(G, drop_rel=1e-14):
+    G = np.asfortranarray(G, dtype=np.complex128)
+    k = G.shape[0]
+    keep = np.zeros(k, dtype=np.int32)
+    Ci = np.zeros((k, k), dtype=np.complex128, order="F")
+    ke = lib().nhf_cholqr_coeffs(k, _dp(G), drop_rel, keep.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(Ci))
+    if ke < 0:
+        _chk(ke)
+    return keep[:ke], np.asfortranarray(Ci.reshape(-1, order="F")[: k * ke].reshape(k, ke, order="F"))
 
 
 def trapezoid_normalize(psi, n, box_size):

@@ -84,7 +84,8 @@ def test_poisson_matches_oracle(H):
 
 def test_scf_o16_matches_oracle(H):
     # O-16 on 16^3 (h = 1.2 fm): converged energies agree to 1e-5 MeV,
-    # single-particle energies to 1e-4 MeV, <r^2> to 1e-6 fm^2
+    # single-particle energies to 1e-4 MeV, <r^2> to 1e-5 fm^2 (both runs
+    # stop at |dE| < 1e-6 MeV, which fixes the states only to ~1e-6 relative)
     hip, ctx = H
     cfg = K.o16_config(n=16, M=8)
     pn, pp, en, ep = K.guess(cfg, ho=O.ho_guess)
@@ -98,5 +99,5 @@ def test_scf_o16_matches_oracle(H):
     assert abs(so["E_HF"] - sg["E_HF"]) < 1e-5
     assert np.abs(r_or[2] - r_gp[2]).max() < 1e-4
     assert np.abs(r_or[3] - r_gp[3]).max() < 1e-4
-    assert abs(r_or[4].r2_n - r_gp[4].r2_n) < 1e-6
-    assert abs(r_or[4].r2_p - r_gp[4].r2_p) < 1e-6
+    assert abs(r_or[4].r2_n - r_gp[4].r2_n) < 1e-5
+    assert abs(r_or[4].r2_p - r_gp[4].r2_p) < 1e-5

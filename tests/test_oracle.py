@@ -1,0 +1,337 @@
+"""Pins for the FP64 CPU oracle (no GPU).  Each check is chosen so that a
+plausible mistake (a dropped term, a wrong sign or index, a transposed
+operand) fails: polynomial exactness, element-by-element assembly, closed
+forms, published nuclear-matter properties, invariants, library routines."""
+import math
+
+import numpy as np
+import pytest
+
+import _cases as K
+import _oracle as O
+from nuclearcgm_amd import config as C
+
+
+# ------------------------------------------------------------ FD stencils
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_first_derivative_polynomial_exactness(axis):
+    # 9-point interior exact to degree 8, 7-point (pos 3, n-4) to degree 6,
+    # 5-point (pos 2, n-3) to 4, 3-point (pos 1, n-2) to 2, 2-point at the
+    # edges to degree 1 (differential_operators.cpp:356-417)
+    n, box = 14, 13.0
+    a, h, X, Y, Z = K.grid(n, box)
+    t = (X, Y, Z)[axis] / 3.0
+    pos = np.round(((X, Y, Z)[axis] + a) / h).astype(int)
+    for deg, ok in [(1, lambda p: np.ones_like(p, bool)), (2, lambda p: (p >= 1) & (p <= n - 2)),
+                    (4, lambda p: (p >= 2) & (p <= n - 3)), (6, lambda p: (p >= 3) & (p <= n - 4)),
+                    (8, lambda p: (p >= 4) & (p <= n - 5))]:
+        f = t ** deg + 0.5 * t ** (deg - 1)
+        df = (deg * t ** (deg - 1) + 0.5 * (deg - 1) * t ** max(deg - 2, 0)) / 3.0
+        got = O.dv_real(n, a, f, axis)
+        m = ok(pos)
+        assert np.abs(got[m] - df[m]).max() < 1e-9 * max(1, np.abs(df).max()), deg
+        if deg == 8:  # and not exact one degree higher in the interior
+            f9 = t ** 9
+            g9 = O.dv_real(n, a, f9, axis)
+            assert np.abs(g9[m] - 9 * t[m] ** 8 / 3.0).max() > 1e-6
+
+
+def test_second_derivative_exactness_and_edge_quirk():
+    n, box = 14, 13.0
+    a, h, X, Y, Z = K.grid(n, box)
+    t = X / 3.0
+    pos = np.round((X + a) / h).astype(int)
+    for deg, lo in [(3, 1), (5, 2), (7, 3), (9, 4)]:
+        f = t ** deg
+        d2 = deg * (deg - 1) * t ** (deg - 2) / 9.0
+        got = O.d2v_real(n, a, f, 0)
+        m = (pos >= lo) & (pos <= n - 1 - lo)
+        assert np.abs(got[m] - d2[m]).max() < 1e-8 * np.abs(d2).max(), deg
+    # pos 0 and n-1 return a first difference (differential_operators.cpp:560-575)
+    f = np.exp(0.1 * X)
+    got = O.d2v_real(n, a, f, 0)
+    e0 = (pos == 0)
+    fd = (np.exp(0.1 * (X + h)) - np.exp(0.1 * X)) / h
+    assert np.allclose(got[e0], fd[e0], rtol=1e-12)
+
+
+def test_spinor_derivative_matches_scalar_derivative():
+    n = 10
+    a, h, X, Y, Z = K.grid(n, 9.0)
+    rng = np.random.default_rng(0)
+    psi = rng.standard_normal(2 * n ** 3) + 1j * rng.standard_normal(2 * n ** 3)
+    for ax in range(3):
+        d = O.dv_spinor(n, a, psi, ax)
+        for s in range(2):
+            for part in (np.real, np.imag):
+                ref = O.dv_real(n, a, np.ascontiguousarray(part(psi[s::2])), ax)
+                assert np.allclose(part(d[s::2]), ref, atol=1e-12)
+
+
+# ------------------------------------------------------------ Hamiltonian
+@pytest.mark.parametrize("n", [8, 10])
+def test_matrix_free_h_equals_elementwise_csr(n):
+    a = 0.5 * 9.0
+    f, gf, U, W = K.random_fields(n, seed=n)
+    H = O.build_csr(n, a, f, gf, U, W, so=True)
+    psi = K.random_states(n, 3, seed=4)
+    y = O.hpsi(n, a, f, gf, U, W, psi)
+    ref = H @ psi
+    assert np.abs(y - ref).max() / np.abs(ref).max() < 1e-14
+    # 1 + 12 same-spin + 12 spin-flip neighbours in the interior (SO on)
+    nnz = np.diff(H.indptr)
+    assert nnz.max() == 25
+
+
+def test_h_hermitian_for_constant_mass_and_so_field():
+    # with constant f and W the row-point H is exactly Hermitian; with a
+    # varying f it is not (the reference's quirk, kept)
+    n, a = 8, 4.0
+    f, gf, U, W = K.random_fields(n, seed=1)
+    fc = np.full_like(f, 20.7)
+    Wc = np.repeat(np.array([0.3, -0.2, 0.5]), n ** 3)
+    H = O.build_csr(n, a, fc, np.zeros_like(gf), U, Wc).toarray()
+    assert np.abs(H - H.conj().T).max() < 1e-12
+    H2 = O.build_csr(n, a, f, gf, U, W).toarray()
+    assert np.abs(H2 - H2.conj().T).max() > 1e-3
+
+
+def test_ho_spectrum_fourth_order():
+    # constant mass + harmonic potential: hbar w (N + 3/2) with 2, 6, 12
+    # degeneracies; the 4th-order lattice error falls ~16x when h halves
+    hw = 10.0
+    errs = []
+    for n, box in [(16, 15.0), (24, 15.0)]:
+        a, h, X, Y, Z = K.grid(n, box)
+        f = np.full(n ** 3, C.HBAR ** 2 / (2 * C.MNUC))
+        U = hw ** 2 * (X ** 2 + Y ** 2 + Z ** 2) / (4 * f)
+        b = C.HBAR / math.sqrt(C.MNUC * hw)
+        X0 = O.ho_guess(n, box, 8, b)
+        prm = C.NhfGcgParams(40, 60, 1e-10, 1e-8)
+        _, lam, _ = O.gcg(n, a, f, np.zeros(3 * n ** 3), U, np.zeros(3 * n ** 3), X0, params=prm)
+        exact = np.array([1.5] * 2 + [2.5] * 6) * hw
+        errs.append(np.abs(lam - exact).max())
+    # h = 1.0 -> 0.652 fm: (1/0.652)^4 = 5.5
+    assert errs[0] < 0.06 and errs[1] < errs[0] / 4.5, errs
+
+
+def test_gcg_matches_dense_eigensolve():
+    # Hermitian case (constant f, constant W, varying U): GCG with enough
+    # passes reproduces the lowest eigenvalues of a dense eigensolve
+    n, a = 8, 5.0
+    f, gf, U, W = K.smooth_fields(n, 10.0)
+    fc = np.full_like(f, 20.7)
+    Wc = np.repeat(np.array([0.4, -0.1, 0.2]), n ** 3)
+    gz = np.zeros_like(gf)
+    H = O.build_csr(n, a, fc, gz, U, Wc).toarray()
+    w = np.linalg.eigvalsh(H)
+    # 8 states = the two lowest Kramers pairs + the 6-fold p-like cluster
+    X0 = K.random_states(n, 8, seed=11)
+    prm = C.NhfGcgParams(60, 40, 1e-9, 1e-8)
+    _, lam, _ = O.gcg(n, a, fc, gz, U, Wc, X0, params=prm)
+    assert np.abs(lam - w[:8]).max() < 1e-9
+
+
+def test_eigh_lower_semantics():
+    # only the lower triangle is read (Eigen SelfAdjointEigenSolver); checked
+    # against numpy's LAPACK on the Hermitian matrix built from it
+    rng = np.random.default_rng(3)
+    k = 37
+    A = rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k))
+    L = np.tril(A, -1)
+    Hm = L + L.conj().T + np.diag(np.diag(A).real)
+    w, V = O.eigh_lower(A)
+    assert np.allclose(w, np.linalg.eigvalsh(Hm), atol=1e-11)
+    assert np.allclose(Hm @ V, V * w, atol=1e-10)
+    assert np.allclose(V.conj().T @ V, np.eye(k), atol=1e-12)
+
+
+def test_eigen_cg_matches_scipy_pcg():
+    # Eigen's ConjugateGradient with the diagonal preconditioner is textbook
+    # PCG; scipy's cg with M = D^-1 produces the same iterates
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    n, a = 8, 5.0
+    f, gf, U, W = K.smooth_fields(n, 10.0)
+    fc = np.full_like(f, 20.7)
+    Wc = np.repeat(np.array([0.4, -0.1, 0.2]), n ** 3)
+    gz = np.zeros_like(gf)
+    shift = 80.0
+    H = O.build_csr(n, a, fc, gz, U, Wc) + shift * sp.identity(2 * n ** 3)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(2 * n ** 3) + 1j * rng.standard_normal(2 * n ** 3)
+    x0 = rng.standard_normal(2 * n ** 3) + 0j
+    dinv = 1.0 / H.diagonal()
+    for iters in (5, 12):
+        x, it = O.cg(n, a, fc, gz, U, Wc, shift, b, x0, iters, 1e-30)
+        xs, _ = sla.cg(H, b, x0=x0, maxiter=iters, M=sp.diags(dinv), rtol=1e-300, atol=0.0)
+        assert np.abs(x.reshape(-1) - xs).max() / np.abs(xs).max() < 1e-10, iters
+
+
+# ------------------------------------------------------------- densities
+def test_densities_plane_wave_gaussian():
+    # psi = exp(i k.r) phi(r) (up): rho = phi^2, tau = k^2 phi^2 + |grad phi|^2,
+    # J_{mu nu} = k_mu phi^2 delta_{nu z}; div J equals the numerical
+    # divergence of the Levi-Civita vector of J
+    n, box = 24, 12.0
+    a, h, X, Y, Z = K.grid(n, box)
+    k = np.array([0.3, -0.2, 0.25])
+    s = 1.6
+    phi = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / (2 * s ** 2))
+    psi = np.zeros((2 * n ** 3, 1), dtype=complex)
+    psi[0::2, 0] = np.exp(1j * (k[0] * X + k[1] * Y + k[2] * Z)) * phi
+    rho, tau, J, divJ = O.densities(n, a, psi, [1.0])
+    assert np.allclose(rho, phi ** 2, atol=1e-14)
+    gphi2 = phi ** 2 * (X ** 2 + Y ** 2 + Z ** 2) / s ** 4
+    inner = (np.abs(X) < a - 4 * h) & (np.abs(Y) < a - 4 * h) & (np.abs(Z) < a - 4 * h)
+    assert np.abs(tau - (k @ k) * phi ** 2 - gphi2)[inner].max() < 2e-4
+    J = J.reshape(9, -1)
+    for mu in range(3):
+        for nu in range(3):
+            ref = k[mu] * phi ** 2 * (1.0 if nu == 2 else 0.0)
+            assert np.abs(J[mu * 3 + nu] - ref)[inner].max() < 2e-4
+    N3 = n ** 3
+    Jv = [J[1 * 3 + 2] - J[2 * 3 + 1], J[2 * 3 + 0] - J[0 * 3 + 2], J[0 * 3 + 1] - J[1 * 3 + 0]]
+    div = sum(O.dv_real(n, a, np.ascontiguousarray(Jv[ax]), ax) for ax in range(3))
+    assert np.abs(div - divJ)[inner].max() < 1e-3 * np.abs(divJ).max()
+    # a spin-orbit-coupled state gives a non-trivial div J (sign check)
+    g = O.ho_guess(n, box, 8, 1.6)
+    occ = np.array([0, 0, 1, 1, 1, 1, 0, 0.0])  # p3/2 m=+-1/2, +-3/2
+    rho, tau, J, divJ = O.densities(n, a, g, occ)
+    J = J.reshape(9, -1)
+    Jv = [J[1 * 3 + 2] - J[2 * 3 + 1], J[2 * 3 + 0] - J[0 * 3 + 2], J[0 * 3 + 1] - J[1 * 3 + 0]]
+    div = sum(O.dv_real(n, a, np.ascontiguousarray(Jv[ax]), ax) for ax in range(3))
+    assert np.abs(div - divJ)[inner].max() < 2e-3 * np.abs(divJ).max()
+    # for a j = l + 1/2 shell J points outwards: int r.J > 0
+    assert float(np.sum(X * Jv[0] + Y * Jv[1] + Z * Jv[2])) > 0
+
+
+# ---------------------------------------------------------------- Coulomb
+def test_poisson_gaussian_closed_form_and_order():
+    # U = Z e^2 erf(r / (sqrt 2 s)) / r for a Gaussian charge; the monopole
+    # boundary is exact for it, so the error is the 4th-order lattice error
+    errs = []
+    for n in (20, 28):
+        box = 16.0
+        a, h, X, Y, Z = K.grid(n, box)
+        r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+        s, Zc = 1.5, 8
+        rho = Zc * np.exp(-r ** 2 / (2 * s ** 2)) / ((2 * math.pi) ** 1.5 * s ** 3)
+        U, _ = O.poisson(n, a, rho, Zc)
+        from scipy.special import erf
+
+        exact = Zc * C_E2 * erf(r / (math.sqrt(2) * s)) / r
+        errs.append(np.abs(U - exact).max() / exact.max())
+    assert errs[0] < 2e-3 and errs[1] < errs[0] / 3, errs
+
+
+C_E2 = 1.43996446
+
+
+def test_poisson_uniform_sphere():
+    # the closed form the reference carries in spherical_coulomb.cpp:9-17
+    n, box = 28, 18.0
+    a, h, X, Y, Z = K.grid(n, box)
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    R, Zc = 4.0, 20
+    inside = r < R
+    rho = np.where(inside, 1.0, 0.0)
+    rho *= Zc / (rho.sum() * h ** 3)
+    U, _ = O.poisson(n, a, rho, Zc)
+    exact = np.where(inside, Zc * C_E2 * (3 - r ** 2 / R ** 2) / (2 * R), Zc * C_E2 / r)
+    assert np.abs(U - exact).max() / exact.max() < 0.03
+
+
+def test_coulomb_boundary_transposition():
+    # ghost values are read at (z, y, x): a prolate (z-elongated) charge gets
+    # the same boundary as its x-elongated mirror image would with the
+    # coordinates exchanged
+    n, box = 12, 12.0
+    a, h, X, Y, Z = K.grid(n, box)
+    rho_z = np.exp(-(X ** 2 + Y ** 2 + 0.3 * Z ** 2) / 3.0)
+    rhs = O.coulomb_rhs(n, a, rho_z, 8)
+    rhs_int = -4 * math.pi * C_E2 * rho_z
+    bnd = rhs - rhs_int
+    # boundary contribution of the face x = -a (i = 0) at (j, k) equals the
+    # reference's value computed with Y20 evaluated at (z_k, y_j, x=-a-h):
+    # check it is not the naive ghost value at (x=-a-h, y_j, z_k)
+    from itertools import product
+
+    q20 = math.sqrt(5 / (16 * math.pi)) * float(np.sum(K.trap_weights(n, box) * rho_z * (2 * Z ** 2 - X ** 2 - Y ** 2)))
+    q22 = math.sqrt(15 / (32 * math.pi)) * float(np.sum(K.trap_weights(n, box) * rho_z * (X ** 2 - Y ** 2)))
+
+    def ubc(x, y, z):
+        r = math.sqrt(x * x + y * y + z * z)
+        ct = z / r
+        y20 = math.sqrt(5 / (16 * math.pi)) * (3 * ct * ct - 1)
+        y22 = math.sqrt(15 / (32 * math.pi)) * math.cos(2 * math.atan2(y, x)) * (1 - ct * ct)
+        return C_E2 * 8 / r + C_E2 * (y20 * q20 + 2 * y22 * q22) / r
+
+    j, k = 5, 3  # face x = -a, away from edges
+    p = 0 + n * (j + n * k)
+    c = lambda t: -a + t * h  # noqa: E731
+    expect = -(16 / 12 * ubc(c(k), c(j), c(-1)) - 1 / 12 * ubc(c(k), c(j), c(-2))) / h ** 2
+    naive = -(16 / 12 * ubc(c(-1), c(j), c(k)) - 1 / 12 * ubc(c(-2), c(j), c(k))) / h ** 2
+    assert abs(bnd[p] - expect) < 1e-10 * abs(expect)
+    assert abs(bnd[p] - naive) > 1e-6 * abs(naive)
+
+
+# ------------------------------------------------------------------- BCS
+def test_bcs_constant_g_gap_and_number_equations():
+    # constant G: the converged state satisfies the gap equation
+    # 1 = (G/2) sum_q w_q / E_q and the number equation sum_p 2 v_p^2 = A
+    eps = np.array([-20.0, -15.0, -12.0, -10.5, -9.0, -8.2, -6.0, -3.0, -1.0, 1.0])
+    P = len(eps)
+    G = np.full((P, P), 1.2)
+    A = 8
+    v2, Delta, lam, Ep = O.solve_bcs(eps, G, A, 250.0, 5.0, True, 0.5 * (eps[3] + eps[4]))
+    assert np.allclose(Delta, Delta[0], rtol=1e-8)
+    ws = 0.5
+    w = 1 / (1 + np.exp((eps - lam - 5) / ws)) / (1 + np.exp(-(eps - lam + 5) / ws))
+    E = np.sqrt((eps - lam) ** 2 + Delta ** 2 * w)
+    assert abs(0.5 * 1.2 * np.sum(w / E) - 1.0) < 1e-8
+    assert abs(2 * v2.sum() - A) < 1e-8
+    # symmetric spectrum at half filling -> lambda = 0
+    eps2 = np.array([-3.0, -2.0, -1.0, 1.0, 2.0, 3.0])
+    v2, Delta, lam, Ep = O.solve_bcs(eps2, np.full((6, 6), 2.0), 6, 250.0, 5.0, True, 0.3)
+    assert abs(lam) < 1e-8 and Ep < 0
+
+
+def test_time_reverse_involution():
+    n = 6
+    psi = K.random_states(n, 2, seed=3)
+    T2 = O.time_reverse(n, O.time_reverse(n, psi))
+    assert np.allclose(T2, -psi)  # T^2 = -1 for spin 1/2
+    g = O.ho_guess(10, 12.0, 8, 1.6)
+    ov = np.abs(np.einsum("ri,ri->i", g[:, 0::2].conj(), O.time_reverse(10, g[:, 1::2])))
+    # +m_j / -m_j partners of the j = l + 1/2 orbitals are exact Kramers pairs
+    assert np.allclose(ov[:3], 1.0, atol=1e-10)
+    # the j = l - 1/2 coefficients of guess.cpp:195-198 use (l + m_j - 1/2)
+    # for the down component (a quirk kept by the restatement): the p1/2
+    # pair overlaps only 1/sqrt(2) after time reversal
+    assert abs(ov[3] - 1 / math.sqrt(2)) < 1e-10
+
+
+# ------------------------------------------------------------------- EDF
+@pytest.mark.parametrize("name,rho0,ea,mstar,asym", [("SLy4", 0.160, -15.97, 0.69, 32.0),
+                                                     ("SKMstar", 0.160, -15.77, 0.79, 30.0)])
+def test_edf_nuclear_matter(name, rho0, ea, mstar, asym):
+    # published saturation properties pin C0rr, C0Drr, C0rt (E/A, m*/m) and
+    # the isovector couplings (symmetry energy)
+    import json, os
+
+    par = json.load(open(os.path.join(C.FUNCTIONAL_DIR, name + ".json")))
+    sk = [par[k] for k in ["t0", "t1", "t2", "t3", "x0", "x1", "x2", "x3", "W0"]] + [par["W0"], 0.0, 0.0,
+                                                                                     par["sigma"]]
+    Cc = O.edf_from_skyrme(sk)
+    assert np.allclose(Cc, C.edf_from_skyrme(par), rtol=1e-14)
+    hb = C.HBAR ** 2 / (2 * C.MNUC)
+    kf = (1.5 * math.pi ** 2 * rho0) ** (1 / 3)
+    tau = 0.6 * kf ** 2 * rho0
+    e = 0.6 * hb * kf ** 2 + Cc[0] * rho0 + Cc[2] * rho0 ** (1 + Cc[12]) + Cc[4] * tau
+    assert abs(e - ea) < 0.15
+    assert abs(hb / (hb + Cc[4] * rho0) - mstar) < 0.02
+    # symmetry energy: kinetic + C1rr rho + C1Drr rho^(1+s) + C1rt-type terms
+    esym = hb * kf ** 2 / 3 + Cc[1] * rho0 + Cc[3] * rho0 ** (1 + Cc[12]) + (Cc[4] / 3 + Cc[5]) * kf ** 2 * rho0
+    assert abs(esym - asym) < 1.0
