@@ -18,6 +18,7 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
     # no packed-FP32 instructions in any HIP code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fcx-limited-range",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value",
 ]
 
