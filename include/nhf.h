@@ -219,6 +219,21 @@ int nhf_scf_run(void *ctx, int pid, double *psi_n, double *psi_p, double *eps_n,
  * No reference counterpart (the reference has no timers on this path). */
 int nhf_bench_hpsi(void *ctx, int pid, int q, int m, int prec, int reps, double *avg_ms);
 
+/* Measurement: live timing of the H.psi applications inside the block-CG
+ * of the GCG steps.  enable = 1 starts (and resets) HIP-event timing of
+ * every CG stencil launch on the species streams, 0 stops it, -1 leaves it;
+ * out (may be NULL): [0] summed kernel ms, [1] launches, [2] active
+ * (state column x launch) count, [3] n^3.  No reference counterpart. */
+int nhf_stencil_stats(void *ctx, int pid, int enable, double *out);
+
+/* Measurement: accumulated wall time (s) of the SCF phases since the last
+ * reset: out[0] shared density/occupation update, out[1] energies, then per
+ * species q (out[2 + 6q + k]): k = 0 fields (+ Coulomb), 1 guess
+ * orthonormalisation and first Rayleigh-Ritz, 2 block CG, 3 basis and
+ * Cholesky-QR, 4 projection + eigensolve + rotation, 5 normalisation /
+ * Kramers order / conjugate direction.  No reference counterpart. */
+int nhf_phase_times(void *ctx, int pid, int reset, double *out);
+
 /* Harmonic-oscillator spinor guess |n l j m_j> with oscillator length b
  * (fm), Euclidean-normalised columns (guess.cpp:10-248 with
  * useSpinOrbit = true).  Host-side; out: 2n^3 x nev complex. */

@@ -147,7 +147,9 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V) {
 // the kept earlier columns is <= drop_rel * G_jj (the column-by-column rule
 // of the modified Gram-Schmidt of solver.cpp:389-416 and :756-764, on a Gram
 // matrix).  Cinv (k x ke, column-major) = rows `keep` of R^-1.
-int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, std::vector<cd>& Cinv) {
+int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, std::vector<cd>& Cinv,
+                  double* min_rel) {
+  double mr = 1.0;
   auto g = [&](int r, int c) -> cd { return r >= c ? G[(size_t)c * k + r] : std::conj(G[(size_t)r * k + c]); };
   keep.clear();
   std::vector<cd> R;  // column-major ke x ke upper triangular, grown column by column
@@ -166,11 +168,13 @@ int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, s
     double gjj = g(j, j).real();
     double dd = gjj - s;
     if (!(dd > drop_rel * gjj) || !(gjj > 0.0)) continue;
+    mr = std::min(mr, dd / gjj);
     rj[ke] = std::sqrt(dd);
     keep.push_back(j);
     cols.push_back(rj);
   }
   const int ke = (int)keep.size();
+  if (min_rel) *min_rel = mr;
   // R^-1 by back substitution, column by column
   std::vector<cd> Ri((size_t)ke * ke, 0.0);
   for (int c = 0; c < ke; ++c) {

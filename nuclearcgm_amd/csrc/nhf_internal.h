@@ -35,27 +35,33 @@ constexpr int CG_NS = 4;
 // spinors has N3 Sp<R> entries; `ld` is the column stride in Sp<R> units.
 template <typename R>
 void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int ncols, const R* fld, int n, double h,
-                 double shift, double* dot, int dot_stride, const int* done);
+                 double shift, double* dot, int dot_stride, const int* done, unsigned long long* act = nullptr,
+                 const int* clist = nullptr);
 
 // block Jacobi-PCG kernels (Eigen ConjugateGradient restated per column)
 template <typename R>
 void launch_cg_init(hipStream_t st, const Sp<R>* X, const double* lam_shift, Sp<R>* W, Sp<R>* Rr, Sp<R>* P,
                     const Sp<R>* T, const R* invd, size_t N3, size_t ld, int ncols, double* s0, double* rhs2);
+// clist (optional): grid.y enumerates clist[0..ncols) instead of 0..ncols
 template <typename R>
 void launch_cg_update(hipStream_t st, Sp<R>* W, Sp<R>* Rr, const Sp<R>* P, const Sp<R>* T, const R* invd, size_t N3,
-                      size_t ld, int ncols, const double* sprev, double* scur, const int* done);
+                      size_t ld, int ncols, const double* sprev, double* scur, const int* done,
+                      const int* clist = nullptr);
 template <typename R>
 void launch_cg_dir(hipStream_t st, const Sp<R>* Rr, Sp<R>* P, const R* invd, size_t N3, size_t ld, int ncols,
-                   const double* sprev, const double* scur, const double* thr, int* done, int* its);
+                   const double* sprev, const double* scur, const double* thr, int* done, int* its,
+                   const int* clist = nullptr);
 void launch_cg_check0(hipStream_t st, const double* s0, const double* rhs2, double tol, int ncols, double* thr,
                       int* done);
 
 // C (ka x kb, complex double, column-major, leading dim ka) = A^H B where
 // A, B are columns of Sp<R> (rows = 2 N3 complex).  part: scratch of
 // nchunk * ka * kb complex doubles.
+// lower != 0 with A == B: only the tiles on or below the diagonal are
+// computed, the rest is filled by Hermitian symmetry
 template <typename R>
 void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>* B, size_t ldb, int kb, size_t N3,
-                 double2* part, int nchunk, double2* C);
+                 double2* part, int nchunk, double2* C, int lower = 0);
 int gram_chunks(size_t N3);
 
 // Y (ncols) = V (k cols) * C (k x ncols complex double, column-major),
@@ -73,6 +79,10 @@ void launch_col_norms(hipStream_t st, const Sp<R>* X, size_t N3, size_t ld, int 
                       double* out);
 template <typename R>
 void launch_col_scale(hipStream_t st, Sp<R>* X, size_t N3, size_t ld, int ncols, const double* s);
+// X_c /= sqrt(w * sum_p weight_p |X_c(p)|^2), weight = trapezoid or 1
+template <typename R>
+void launch_col_normalize(hipStream_t st, Sp<R>* X, size_t N3, size_t ld, int ncols, int n, int trapezoid, double w,
+                          double* norms);
 template <typename Ra, typename Rb>
 void launch_convert(hipStream_t st, const Sp<Ra>* a, Sp<Rb>* b, size_t count);
 template <typename R>
@@ -97,11 +107,12 @@ struct FieldParams {
 };
 // derived fields of both species from the mixed densities: writes the
 // packed stencil fields fld64[q] (and fld32[q] when non-null), grad rho,
-// lap rho, J vector (3 planes) and the Coulomb potential U_C.
+// lap rho, J vector (3 planes) and the Coulomb potential U_C, for the
+// species in qmask (bit 0 neutrons, bit 1 protons; U_C with the protons).
 void launch_fields(hipStream_t st, const FieldParams& P, int n, double a, double h, const double* const rho[2],
                    const double* const tau[2], const double* const J9[2], const double* const divJ[2],
                    const double* UCdir, double* grho[2], double* lrho[2], double* Jv[2], double* UCoul,
-                   double* fld64[2], float* fld32[2]);
+                   double* fld64[2], float* fld32[2], int qmask);
 
 // weighted reductions: out[k] = int O_k(r) rho_tot(r) d^3r for the
 // constraint operators kind[k] (plus Q20/Q22 of the proton density for the
@@ -118,6 +129,15 @@ void launch_poisson_update(hipStream_t st, double* U, double* r, const double* p
 void launch_poisson_dir(hipStream_t st, const double* r, double* p, size_t N3, const double* scal, int slot);
 void launch_poisson_init(hipStream_t st, const double* rhs, const double* AU, double* r, double* p, size_t N3,
                          double* bn_rr);
+
+// DST-I fast-diagonalisation preconditioner of the 7-point Dirichlet
+// Laplacian (S: n x n orthonormal DST-I matrix on the device)
+void launch_dst_precond(hipStream_t st, const double* r, double* z, double* tmp, const double* S, int n, double h);
+void launch_dot(hipStream_t st, const double* a, const double* b, size_t N3, double* out);
+// preconditioned-CG pieces with scalars scal[4 slot + {0 pAp, 1 rr, 2 rz}]
+void launch_pcg_update(hipStream_t st, double* U, double* r, const double* p, const double* Ap, size_t N3,
+                       const double* scal, int slot, double* rr);
+void launch_pcg_dir(hipStream_t st, const double* z, double* p, size_t N3, const double* scal, int slot);
 
 // energy integrands -> NHF energy term partial sums (trapezoid, FP64)
 void launch_energy(hipStream_t st, const FieldParams& P, int n, double h, const double* const rho[2],
@@ -140,6 +160,8 @@ bool eigh_lower(int k, const cd* A, double* w, cd* V);
 // (k x k Hermitian, lower triangle used) return the kept column list and
 // Cinv (k x ke) with V_eff = V * Cinv orthonormal.  drop_rel: a column is
 // dropped when its new component has squared norm <= drop_rel * G_jj.
-int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, std::vector<cd>& Cinv);
+// min_rel (optional): smallest kept d_jj / G_jj, a conditioning measure.
+int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, std::vector<cd>& Cinv,
+                  double* min_rel = nullptr);
 
 }  // namespace nhf

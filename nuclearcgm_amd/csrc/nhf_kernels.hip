@@ -8,11 +8,24 @@
 // 64 consecutive points = one coalesced 1 KiB (FP32) / 2 KiB (FP64) segment.
 #include "nhf_internal.h"
 
+#include <algorithm>
 #include <cfloat>
+#include <cstdlib>
+#include <string>
 
 namespace nhf {
 
 constexpr int BS = 256;
+
+// stencil variant: 1 = 2.5D LDS/register kernel (default), 0 = direct loads
+// (environment NHF_HPSI=direct)
+static int hpsi_variant() {
+  static int v = [] {
+    const char* e = getenv("NHF_HPSI");
+    return (e && std::string(e) == "direct") ? 0 : 1;
+  }();
+  return v;
+}
 
 // ------------------------------------------------------------- helpers
 template <typename R>
@@ -44,6 +57,18 @@ __device__ __forceinline__ void block_atomic_add(double (&v)[NV], double* dst) {
   }
 }
 
+// the 8 packed field scalars of a point with 16-byte vector loads
+__device__ __forceinline__ void load_fields(const float* __restrict__ fld, size_t p, float (&F)[NFLD]) {
+  const float4* f4 = reinterpret_cast<const float4*>(fld + (size_t)NFLD * p);
+  const float4 a = f4[0], b = f4[1];
+  F[0] = a.x; F[1] = a.y; F[2] = a.z; F[3] = a.w; F[4] = b.x; F[5] = b.y; F[6] = b.z; F[7] = b.w;
+}
+__device__ __forceinline__ void load_fields(const double* __restrict__ fld, size_t p, double (&F)[NFLD]) {
+  const double2* f2 = reinterpret_cast<const double2*>(fld + (size_t)NFLD * p);
+  const double2 a = f2[0], b = f2[1], c = f2[2], d = f2[3];
+  F[0] = a.x; F[1] = a.y; F[2] = b.x; F[3] = b.y; F[4] = c.x; F[5] = c.y; F[6] = d.x; F[7] = d.y;
+}
+
 __device__ __forceinline__ void decompose(size_t p, int n, int& i, int& j, int& k) {
   const int nn = n * n;
   k = (int)(p / (size_t)nn);
@@ -70,9 +95,11 @@ template <typename R, bool DOT>
 __global__ __launch_bounds__(BS) void k_hpsi(const Sp<R>* __restrict__ in, Sp<R>* __restrict__ out, size_t ld,
                                              const R* __restrict__ fld, int n, size_t N3, R c1, R c2, R shift,
                                              double* __restrict__ dot, int dot_stride,
-                                             const int* __restrict__ done) {
-  const int c = blockIdx.y;
+                                             const int* __restrict__ done, unsigned long long* __restrict__ act,
+                                             const int* __restrict__ clist) {
+  const int c = clist ? clist[blockIdx.y] : (int)blockIdx.y;
   if (done && done[c]) return;  // uniform over the workgroup
+  if (act && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(act, 1ull);
   const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
   const Sp<R>* __restrict__ src = in + (size_t)c * ld;
   double acc[2] = {0.0, 0.0};
@@ -110,7 +137,8 @@ __global__ __launch_bounds__(BS) void k_hpsi(const Sp<R>* __restrict__ in, Sp<R>
     lap.ui = (R(16) * s16.ui - s1.ui - R(90) * a0.ui) * c2;
     lap.dr = (R(16) * s16.dr - s1.dr - R(90) * a0.dr) * c2;
     lap.di = (R(16) * s16.di - s1.di - R(90) * a0.di) * c2;
-    const R* __restrict__ F = fld + (size_t)NFLD * p;
+    R F[NFLD];
+    load_fields(fld, p, F);
     const R f = F[0], gx = F[1], gy = F[2], gz = F[3], U = F[4] + shift, Wx = F[5], Wy = F[6], Wz = F[7];
     // T_0 = -Wy dx0 - i Wz dx1 + Wx dy0 - Wz dy1 + (Wy + i Wx) dz1
     // T_1 = i Wz dx0 + Wy dx1 - Wz dy0 - Wx dy1 + (Wy - i Wx) dz0
@@ -134,23 +162,151 @@ __global__ __launch_bounds__(BS) void k_hpsi(const Sp<R>* __restrict__ in, Sp<R>
   if (DOT) block_atomic_add<2>(acc, dot + (size_t)dot_stride * c);
 }
 
+// 2.5D variant: a workgroup owns TP = 256 consecutive points q = x + n y of
+// an x-y plane and marches over a chunk of z planes.  The point's own z
+// neighbours live in a 5-deep register queue, the plane tile plus a halo of
+// two rows on each side is staged in LDS, so each spinor point is read from
+// global memory ~1 + 4n/TP times instead of 13.
+template <typename R, bool DOT>
+__global__ __launch_bounds__(BS) void k_hpsi25(const Sp<R>* __restrict__ in, Sp<R>* __restrict__ out, size_t ld,
+                                               const R* __restrict__ fld, int n, R c1, R c2, R shift, int zc,
+                                               double* __restrict__ dot, int dot_stride,
+                                               const int* __restrict__ done, unsigned long long* __restrict__ act,
+                                               const int* __restrict__ clist) {
+  extern __shared__ char smem[];
+  Sp<R>* tile = reinterpret_cast<Sp<R>*>(smem);  // BS + 4n entries
+  const int c = clist ? clist[blockIdx.z] : (int)blockIdx.z;
+  if (done && done[c]) return;  // uniform over the workgroup
+  if (act && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) atomicAdd(act, 1ull);
+  const int nn = n * n;
+  const int tid = threadIdx.x;
+  const int q0 = blockIdx.x * BS;
+  const int q = q0 + tid;
+  const bool own = q < nn;
+  const int y = own ? q / n : 0;
+  const int x = own ? q - y * n : 0;
+  const int z0 = blockIdx.y * zc;
+  const int z1 = min(n, z0 + zc);
+  const size_t N3 = (size_t)nn * n;
+  const Sp<R>* __restrict__ src = in + (size_t)c * ld;
+  Sp<R>* __restrict__ dst = out + (size_t)c * ld;
+  const Sp<R> Z = sp_zero<R>();
+  auto ldz = [&](int z) -> Sp<R> { return (own && z >= 0 && z < n) ? src[(size_t)z * nn + q] : Z; };
+  Sp<R> zm2 = ldz(z0 - 2), zm1 = ldz(z0 - 1), z00 = ldz(z0), zp1 = ldz(z0 + 1);
+  // halo: entries [0, 2n) hold q0-2n .. q0-1, [2n+BS, 4n+BS) hold q0+BS .. q0+BS+2n-1
+  const int nh = 4 * n;
+  int hq = -1, hslot = -1;
+  if (tid < nh) {
+    if (tid < 2 * n) {
+      hq = q0 - 2 * n + tid;
+      hslot = tid;
+    } else {
+      hq = q0 + BS + (tid - 2 * n);
+      hslot = BS + tid;
+    }
+    if (hq < 0 || hq >= nn) hq = -1;
+  }
+  double acc[2] = {0.0, 0.0};
+  for (int z = z0; z < z1; ++z) {
+    const Sp<R> zp2 = ldz(z + 2);
+    tile[2 * n + tid] = z00;
+    if (hslot >= 0) tile[hslot] = hq >= 0 ? src[(size_t)z * nn + hq] : Z;
+    __syncthreads();
+    if (own) {
+      const int ct = 2 * n + tid;
+      const Sp<R> xm1 = x >= 1 ? tile[ct - 1] : Z;
+      const Sp<R> xm2 = x >= 2 ? tile[ct - 2] : Z;
+      const Sp<R> xp1 = x + 1 < n ? tile[ct + 1] : Z;
+      const Sp<R> xp2 = x + 2 < n ? tile[ct + 2] : Z;
+      const Sp<R> ym1 = y >= 1 ? tile[ct - n] : Z;
+      const Sp<R> ym2 = y >= 2 ? tile[ct - 2 * n] : Z;
+      const Sp<R> yp1 = y + 1 < n ? tile[ct + n] : Z;
+      const Sp<R> yp2 = y + 2 < n ? tile[ct + 2 * n] : Z;
+      const Sp<R> a0 = z00;
+      Sp<R> d[3];
+#define NHF_D(dd, P1, M1, P2, M2)                                  \
+  dd.ur = (R(8) * (P1.ur - M1.ur) - (P2.ur - M2.ur)) * c1;         \
+  dd.ui = (R(8) * (P1.ui - M1.ui) - (P2.ui - M2.ui)) * c1;         \
+  dd.dr = (R(8) * (P1.dr - M1.dr) - (P2.dr - M2.dr)) * c1;         \
+  dd.di = (R(8) * (P1.di - M1.di) - (P2.di - M2.di)) * c1;
+      NHF_D(d[0], xp1, xm1, xp2, xm2)
+      NHF_D(d[1], yp1, ym1, yp2, ym2)
+      NHF_D(d[2], zp1, zm1, zp2, zm2)
+#undef NHF_D
+      Sp<R> lap;
+      lap.ur = (R(16) * (xp1.ur + xm1.ur + yp1.ur + ym1.ur + zp1.ur + zm1.ur) -
+                (xp2.ur + xm2.ur + yp2.ur + ym2.ur + zp2.ur + zm2.ur) - R(90) * a0.ur) * c2;
+      lap.ui = (R(16) * (xp1.ui + xm1.ui + yp1.ui + ym1.ui + zp1.ui + zm1.ui) -
+                (xp2.ui + xm2.ui + yp2.ui + ym2.ui + zp2.ui + zm2.ui) - R(90) * a0.ui) * c2;
+      lap.dr = (R(16) * (xp1.dr + xm1.dr + yp1.dr + ym1.dr + zp1.dr + zm1.dr) -
+                (xp2.dr + xm2.dr + yp2.dr + ym2.dr + zp2.dr + zm2.dr) - R(90) * a0.dr) * c2;
+      lap.di = (R(16) * (xp1.di + xm1.di + yp1.di + ym1.di + zp1.di + zm1.di) -
+                (xp2.di + xm2.di + yp2.di + ym2.di + zp2.di + zm2.di) - R(90) * a0.di) * c2;
+      const size_t p = (size_t)z * nn + q;
+      R F[NFLD];
+      load_fields(fld, p, F);
+      const R f = F[0], gx = F[1], gy = F[2], gz = F[3], U = F[4] + shift, Wx = F[5], Wy = F[6], Wz = F[7];
+      const R t0r = -Wy * d[0].ur + Wz * d[0].di + Wx * d[1].ur - Wz * d[1].dr + Wy * d[2].dr - Wx * d[2].di;
+      const R t0i = -Wy * d[0].ui - Wz * d[0].dr + Wx * d[1].ui - Wz * d[1].di + Wy * d[2].di + Wx * d[2].dr;
+      const R t1r = -Wz * d[0].ui + Wy * d[0].dr - Wz * d[1].ur - Wx * d[1].dr + Wy * d[2].ur + Wx * d[2].ui;
+      const R t1i = Wz * d[0].ur + Wy * d[0].di - Wz * d[1].ui - Wx * d[1].di + Wy * d[2].ui - Wx * d[2].ur;
+      Sp<R> o;
+      o.ur = -f * lap.ur - (gx * d[0].ur + gy * d[1].ur + gz * d[2].ur) + U * a0.ur + t0i;
+      o.ui = -f * lap.ui - (gx * d[0].ui + gy * d[1].ui + gz * d[2].ui) + U * a0.ui - t0r;
+      o.dr = -f * lap.dr - (gx * d[0].dr + gy * d[1].dr + gz * d[2].dr) + U * a0.dr + t1i;
+      o.di = -f * lap.di - (gx * d[0].di + gy * d[1].di + gz * d[2].di) + U * a0.di - t1r;
+      dst[p] = o;
+      if (DOT) {
+        acc[0] += (double)a0.ur * o.ur + (double)a0.ui * o.ui + (double)a0.dr * o.dr + (double)a0.di * o.di;
+        acc[1] += (double)a0.ur * o.ui - (double)a0.ui * o.ur + (double)a0.dr * o.di - (double)a0.di * o.dr;
+      }
+    }
+    __syncthreads();
+    zm2 = zm1;
+    zm1 = z00;
+    z00 = zp1;
+    zp1 = zp2;
+  }
+  (void)N3;
+  if (DOT) block_atomic_add<2>(acc, dot + (size_t)dot_stride * c);
+}
+
 template <typename R>
 void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int ncols, const R* fld, int n, double h,
-                 double shift, double* dot, int dot_stride, const int* done) {
+                 double shift, double* dot, int dot_stride, const int* done, unsigned long long* act,
+                 const int* clist) {
   size_t N3 = (size_t)n * n * n;
-  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
   R c1 = (R)(1.0 / (12.0 * h)), c2 = (R)(1.0 / (12.0 * h * h));
+  if (4 * n <= BS && hpsi_variant() == 1) {
+    // z chunks: enough workgroups to fill the device (>= ~2048), but no
+    // chunk shorter than 4 planes (4 warm-up loads per chunk)
+    const int ntile = (n * n + BS - 1) / BS;
+    int nzc = (2048 + ntile * ncols - 1) / (ntile * ncols);
+    nzc = std::max(1, std::min(nzc, n / 4));
+    const int zc = (n + nzc - 1) / nzc;
+    nzc = (n + zc - 1) / zc;
+    dim3 g3((unsigned)ntile, (unsigned)nzc, (unsigned)ncols);
+    size_t sh = sizeof(Sp<R>) * (BS + 4 * n);
+    if (dot)
+      hipLaunchKernelGGL((k_hpsi25<R, true>), g3, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
+                         dot_stride, done, act, clist);
+    else
+      hipLaunchKernelGGL((k_hpsi25<R, false>), g3, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
+                         dot_stride, done, act, clist);
+    return;
+  }
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
   if (dot)
     hipLaunchKernelGGL((k_hpsi<R, true>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot,
-                       dot_stride, done);
+                       dot_stride, done, act, clist);
   else
     hipLaunchKernelGGL((k_hpsi<R, false>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot,
-                       dot_stride, done);
+                       dot_stride, done, act, clist);
 }
 template void launch_hpsi<double>(hipStream_t, const Sp<double>*, Sp<double>*, size_t, int, const double*, int, double,
-                                  double, double*, int, const int*);
+                                  double, double*, int, const int*, unsigned long long*, const int*);
 template void launch_hpsi<float>(hipStream_t, const Sp<float>*, Sp<float>*, size_t, int, const float*, int, double,
-                                 double, double*, int, const int*);
+                                 double, double*, int, const int*, unsigned long long*, const int*);
 
 // ======================================================== block Jacobi-PCG
 // Per column c the iteration of Eigen's ConjugateGradient with the diagonal
@@ -216,8 +372,8 @@ __global__ __launch_bounds__(BS) void k_cg_update(Sp<R>* __restrict__ W, Sp<R>* 
                                                   const Sp<R>* __restrict__ P, const Sp<R>* __restrict__ T,
                                                   const R* __restrict__ invd, size_t N3, size_t ld,
                                                   const double* __restrict__ sprev, double* __restrict__ scur,
-                                                  const int* __restrict__ done) {
-  const int c = blockIdx.y;
+                                                  const int* __restrict__ done, const int* __restrict__ clist) {
+  const int c = clist ? clist[blockIdx.y] : (int)blockIdx.y;
   if (done[c]) return;
   const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
   const double an = sprev[(size_t)c * CG_NS + 3];
@@ -247,9 +403,9 @@ __global__ __launch_bounds__(BS) void k_cg_update(Sp<R>* __restrict__ W, Sp<R>* 
 }
 template <typename R>
 void launch_cg_update(hipStream_t st, Sp<R>* W, Sp<R>* Rr, const Sp<R>* P, const Sp<R>* T, const R* invd, size_t N3,
-                      size_t ld, int ncols, const double* sprev, double* scur, const int* done) {
+                      size_t ld, int ncols, const double* sprev, double* scur, const int* done, const int* clist) {
   dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
-  hipLaunchKernelGGL((k_cg_update<R>), grid, dim3(BS), 0, st, W, Rr, P, T, invd, N3, ld, sprev, scur, done);
+  hipLaunchKernelGGL((k_cg_update<R>), grid, dim3(BS), 0, st, W, Rr, P, T, invd, N3, ld, sprev, scur, done, clist);
 }
 
 // convergence test |r|^2 < thr, else p = z + beta p
@@ -258,8 +414,8 @@ __global__ __launch_bounds__(BS) void k_cg_dir(const Sp<R>* __restrict__ Rr, Sp<
                                                const R* __restrict__ invd, size_t N3, size_t ld,
                                                const double* __restrict__ sprev, const double* __restrict__ scur,
                                                const double* __restrict__ thr, int* __restrict__ done,
-                                               int* __restrict__ its) {
-  const int c = blockIdx.y;
+                                               int* __restrict__ its, const int* __restrict__ clist) {
+  const int c = clist ? clist[blockIdx.y] : (int)blockIdx.y;
   if (done[c]) return;
   const bool first = blockIdx.x == 0 && threadIdx.x == 0;
   if (scur[(size_t)c * CG_NS + 2] < thr[c]) {
@@ -283,18 +439,19 @@ __global__ __launch_bounds__(BS) void k_cg_dir(const Sp<R>* __restrict__ Rr, Sp<
 }
 template <typename R>
 void launch_cg_dir(hipStream_t st, const Sp<R>* Rr, Sp<R>* P, const R* invd, size_t N3, size_t ld, int ncols,
-                   const double* sprev, const double* scur, const double* thr, int* done, int* its) {
+                   const double* sprev, const double* scur, const double* thr, int* done, int* its,
+                   const int* clist) {
   dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
-  hipLaunchKernelGGL((k_cg_dir<R>), grid, dim3(BS), 0, st, Rr, P, invd, N3, ld, sprev, scur, thr, done, its);
+  hipLaunchKernelGGL((k_cg_dir<R>), grid, dim3(BS), 0, st, Rr, P, invd, N3, ld, sprev, scur, thr, done, its, clist);
 }
 
 #define INST_CG(R)                                                                                                    \
   template void launch_cg_init<R>(hipStream_t, const Sp<R>*, const double*, Sp<R>*, Sp<R>*, Sp<R>*, const Sp<R>*,    \
                                   const R*, size_t, size_t, int, double*, double*);                                  \
   template void launch_cg_update<R>(hipStream_t, Sp<R>*, Sp<R>*, const Sp<R>*, const Sp<R>*, const R*, size_t,       \
-                                    size_t, int, const double*, double*, const int*);                                \
+                                    size_t, int, const double*, double*, const int*, const int*);                    \
   template void launch_cg_dir<R>(hipStream_t, const Sp<R>*, Sp<R>*, const R*, size_t, size_t, int, const double*,    \
-                                 const double*, const double*, int*, int*);
+                                 const double*, const double*, int*, int*, const int*);
 INST_CG(double)
 INST_CG(float)
 
@@ -329,11 +486,12 @@ int gram_chunks(size_t N3) { return (int)((N3 + GRAM_CHUNK - 1) / GRAM_CHUNK); }
 template <typename R>
 __global__ __launch_bounds__(BS) void k_gram(const Sp<R>* __restrict__ A, size_t lda, int ka,
                                              const Sp<R>* __restrict__ B, size_t ldb, int kb, size_t N3,
-                                             double2* __restrict__ part) {
+                                             double2* __restrict__ part, int lower) {
   __shared__ double2 As[2 * GROWS][GT + 1];
   __shared__ double2 Bs[2 * GROWS][GT + 1];
   const int ntj = (kb + GT - 1) / GT;
   const int ti0 = (blockIdx.x / ntj) * GT, tj0 = (blockIdx.x % ntj) * GT;
+  if (lower && ti0 < tj0) return;  // tile strictly above the diagonal
   const int chunk = blockIdx.y;
   const size_t p0 = (size_t)chunk * GRAM_CHUNK;
   const size_t p1 = min(N3, p0 + GRAM_CHUNK);
@@ -368,30 +526,38 @@ __global__ __launch_bounds__(BS) void k_gram(const Sp<R>* __restrict__ A, size_t
   if (i < ka && j < kb) part[((size_t)chunk * kb + j) * ka + i] = make_double2(sr, si);
 }
 
-__global__ void k_reduce_parts(const double2* __restrict__ part, int nchunk, size_t cnt, double2* __restrict__ C) {
+// sums the chunk partials; with `lower` (Hermitian result) the entries of
+// skipped upper tiles are taken as the conjugate of their mirror
+__global__ void k_reduce_parts(const double2* __restrict__ part, int nchunk, size_t cnt, int ka, int lower,
+                               double2* __restrict__ C) {
   size_t e = (size_t)blockIdx.x * BS + threadIdx.x;
   if (e >= cnt) return;
+  const int i = (int)(e % ka), j = (int)(e / ka);
+  const bool mirror = lower && (i / GT) < (j / GT);
+  const size_t src = mirror ? (size_t)i * ka + j : e;
   double sr = 0.0, si = 0.0;
   for (int c = 0; c < nchunk; ++c) {
-    double2 v = part[(size_t)c * cnt + e];
+    double2 v = part[(size_t)c * cnt + src];
     sr += v.x;
     si += v.y;
   }
-  C[e] = make_double2(sr, si);
+  C[e] = mirror ? make_double2(sr, -si) : make_double2(sr, si);
 }
 
 template <typename R>
 void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>* B, size_t ldb, int kb, size_t N3,
-                 double2* part, int nchunk, double2* C) {
+                 double2* part, int nchunk, double2* C, int lower) {
   int nti = (ka + GT - 1) / GT, ntj = (kb + GT - 1) / GT;
-  hipLaunchKernelGGL((k_gram<R>), dim3(nti * ntj, nchunk), dim3(BS), 0, st, A, lda, ka, B, ldb, kb, N3, part);
+  lower = (lower && A == B && ka == kb) ? 1 : 0;
+  hipLaunchKernelGGL((k_gram<R>), dim3(nti * ntj, nchunk), dim3(BS), 0, st, A, lda, ka, B, ldb, kb, N3, part, lower);
   size_t cnt = (size_t)ka * kb;
-  hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((cnt + BS - 1) / BS)), dim3(BS), 0, st, part, nchunk, cnt, C);
+  hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((cnt + BS - 1) / BS)), dim3(BS), 0, st, part, nchunk, cnt, ka,
+                     lower, C);
 }
 template void launch_gram<double>(hipStream_t, const Sp<double>*, size_t, int, const Sp<double>*, size_t, int, size_t,
-                                  double2*, int, double2*);
+                                  double2*, int, double2*, int);
 template void launch_gram<float>(hipStream_t, const Sp<float>*, size_t, int, const Sp<float>*, size_t, int, size_t,
-                                 double2*, int, double2*);
+                                 double2*, int, double2*, int);
 
 // Y[:, c] (+)= sum_l V[:, l] C[l, c]; optional residual norms
 // |AV c - w_c V c|^2 (the Ritz residual of column c).
@@ -553,6 +719,30 @@ void launch_col_scale(hipStream_t st, Sp<R>* X, size_t N3, size_t ld, int ncols,
 }
 template void launch_col_scale<double>(hipStream_t, Sp<double>*, size_t, size_t, int, const double*);
 template void launch_col_scale<float>(hipStream_t, Sp<float>*, size_t, size_t, int, const double*);
+
+// X_c *= 1 / sqrt(norms[c] * w) (0 for a zero column): the scale from a
+// device-side norm, no host round trip
+template <typename R>
+__global__ void k_col_unit(Sp<R>* __restrict__ X, size_t N3, size_t ld, const double* __restrict__ norms, double w) {
+  const int c = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  const double nv = norms[c] * w;
+  const R f = (R)(nv > 0.0 ? 1.0 / sqrt(nv) : 0.0);
+  const size_t o = (size_t)c * ld + p;
+  Sp<R> x = X[o];
+  X[o] = Sp<R>{x.ur * f, x.ui * f, x.dr * f, x.di * f};
+}
+template <typename R>
+void launch_col_normalize(hipStream_t st, Sp<R>* X, size_t N3, size_t ld, int ncols, int n, int trapezoid, double w,
+                          double* norms) {
+  (void)hipMemsetAsync(norms, 0, sizeof(double) * ncols, st);
+  launch_col_norms<R>(st, X, N3, ld, ncols, n, trapezoid, norms);
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  hipLaunchKernelGGL((k_col_unit<R>), grid, dim3(BS), 0, st, X, N3, ld, norms, w);
+}
+template void launch_col_normalize<double>(hipStream_t, Sp<double>*, size_t, size_t, int, int, int, double, double*);
+template void launch_col_normalize<float>(hipStream_t, Sp<float>*, size_t, size_t, int, int, int, double, double*);
 
 template <typename Ra, typename Rb>
 __global__ void k_convert(const Sp<Ra>* __restrict__ a, Sp<Rb>* __restrict__ b, size_t cnt) {
@@ -748,7 +938,7 @@ struct FieldArgs {
   float* fld32[2];
 };
 
-__global__ __launch_bounds__(BS) void k_fields(FieldArgs A, int n, double a, double h) {
+__global__ __launch_bounds__(BS) void k_fields(FieldArgs A, int n, double a, double h, int qmask) {
   const size_t N3 = (size_t)n * n * n;
   const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
   if (p >= N3) return;
@@ -761,6 +951,7 @@ __global__ __launch_bounds__(BS) void k_fields(FieldArgs A, int n, double a, dou
   for (int q = 0; q < 2; ++q) {
     const double* rq = A.rho[q];
     lr[q] = 0.0;
+    const bool own = (qmask >> q) & 1;
     for (int ax = 0; ax < 3; ++ax) {
       const size_t base = p - (size_t)pos[ax] * stride[ax];
       const size_t s = stride[ax];
@@ -773,11 +964,13 @@ __global__ __launch_bounds__(BS) void k_fields(FieldArgs A, int n, double a, dou
     jv[q][0] = J[(1 * 3 + 2) * N3 + p] - J[(2 * 3 + 1) * N3 + p];
     jv[q][1] = J[(2 * 3 + 0) * N3 + p] - J[(0 * 3 + 2) * N3 + p];
     jv[q][2] = J[(0 * 3 + 1) * N3 + p] - J[(1 * 3 + 0) * N3 + p];
-    for (int ax = 0; ax < 3; ++ax) {
-      A.grho[q][ax * N3 + p] = gr[q][ax];
-      A.Jv[q][ax * N3 + p] = jv[q][ax];
+    if (own) {
+      for (int ax = 0; ax < 3; ++ax) {
+        A.grho[q][ax * N3 + p] = gr[q][ax];
+        A.Jv[q][ax * N3 + p] = jv[q][ax];
+      }
+      A.lrho[q][p] = lr[q];
     }
-    A.lrho[q][p] = lr[q];
   }
   const double rho = A.rho[0][p] + A.rho[1][p];
   const double tau = A.tau[0][p] + A.tau[1][p];
@@ -790,10 +983,11 @@ __global__ __launch_bounds__(BS) void k_fields(FieldArgs A, int n, double a, dou
   double ucon = 0.0;
   for (int c = 0; c < A.P.ncon; ++c) ucon += A.P.con_coef[c] * con_op(A.P.con_kind[c], x, y, z);
   double ucoul = 0.0;
-  if (A.P.coulomb) ucoul = A.UCdir[p] - E2 * cbrt(3.0 / PI) * cbrt(A.rho[1][p]);
-  A.UCoul[p] = ucoul;
+  if (A.P.coulomb && (qmask & 2)) ucoul = A.UCdir[p] - E2 * cbrt(3.0 / PI) * cbrt(A.rho[1][p]);
+  if (qmask & 2) A.UCoul[p] = ucoul;
   const double rs = rho > 0.0 ? pow(rho, sig + 1.0) : 0.0;
   for (int q = 0; q < 2; ++q) {
+    if (!((qmask >> q) & 1)) continue;
     const double rq = A.rho[q][p], tq = A.tau[q][p], lq = lr[q], djq = q == 0 ? dJn : dJp;
     // EffectiveMass (effective_mass.cpp:7-34)
     const double f = A.P.hb2m + (C[NHF_EDF_C0RT] - C[NHF_EDF_C1RT]) * rho + 2 * C[NHF_EDF_C1RT] * rq;
@@ -840,7 +1034,7 @@ __global__ __launch_bounds__(BS) void k_fields(FieldArgs A, int n, double a, dou
 void launch_fields(hipStream_t st, const FieldParams& P, int n, double a, double h, const double* const rho[2],
                    const double* const tau[2], const double* const J9[2], const double* const divJ[2],
                    const double* UCdir, double* grho[2], double* lrho[2], double* Jv[2], double* UCoul,
-                   double* fld64[2], float* fld32[2]) {
+                   double* fld64[2], float* fld32[2], int qmask) {
   FieldArgs A;
   A.P = P;
   for (int q = 0; q < 2; ++q) {
@@ -857,7 +1051,7 @@ void launch_fields(hipStream_t st, const FieldParams& P, int n, double a, double
   A.UCdir = UCdir;
   A.UCoul = UCoul;
   size_t N3 = (size_t)n * n * n;
-  hipLaunchKernelGGL(k_fields, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, A, n, a, h);
+  hipLaunchKernelGGL(k_fields, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, A, n, a, h, qmask);
 }
 
 // moments of the total density for the constraints and Q20/Q22 of the
@@ -1025,6 +1219,96 @@ __global__ __launch_bounds__(BS) void k_poisson_init(const double* __restrict__ 
 void launch_poisson_init(hipStream_t st, const double* rhs, const double* AU, double* r, double* p, size_t N3,
                          double* bn_rr) {
   hipLaunchKernelGGL(k_poisson_init, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, rhs, AU, r, p, N3, bn_rr);
+}
+
+// ------------------------------------------------ DST preconditioner
+// M^-1 = S3 diag(1/lambda) S3 with S the orthonormal DST-I matrix
+// S_jk = sqrt(2/(n+1)) sin(pi (j+1)(k+1)/(n+1)) (S^2 = I) applied along each
+// axis, lambda_ijk = (4/h^2) [sin^2(pi(i+1)/(2(n+1))) + (j) + (k)]: the exact
+// inverse of the 7-point Laplacian with zero Dirichlet data, spectrally
+// equivalent to the 13-point operator (kappa ~ 4/3).
+__global__ __launch_bounds__(BS) void k_axis_apply(const double* __restrict__ in, double* __restrict__ out,
+                                                   const double* __restrict__ S, int n, int axis) {
+  extern __shared__ double Ss[];  // n x n
+  for (int e = threadIdx.x; e < n * n; e += BS) Ss[e] = S[e];
+  __syncthreads();
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  int i, j, k;
+  decompose(p, n, i, j, k);
+  const int pos = axis == 0 ? i : axis == 1 ? j : k;
+  const size_t stride = axis == 0 ? 1 : axis == 1 ? (size_t)n : (size_t)n * n;
+  const size_t base = p - (size_t)pos * stride;
+  double s = 0.0;
+  for (int t = 0; t < n; ++t) s += Ss[pos * n + t] * in[base + (size_t)t * stride];
+  out[p] = s;
+}
+__global__ void k_dst_scale(double* __restrict__ x, int n, double h) {
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  int i, j, k;
+  decompose(p, n, i, j, k);
+  const double c = PI / (2.0 * (n + 1));
+  const double si = sin(c * (i + 1)), sj = sin(c * (j + 1)), sk = sin(c * (k + 1));
+  x[p] /= (4.0 / (h * h)) * (si * si + sj * sj + sk * sk);
+}
+// z = M^-1 r (tmp: scratch of n^3)
+void launch_dst_precond(hipStream_t st, const double* r, double* z, double* tmp, const double* S, int n, double h) {
+  const size_t N3 = (size_t)n * n * n;
+  const dim3 g((unsigned)((N3 + BS - 1) / BS));
+  const size_t sh = sizeof(double) * n * n;
+  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, r, z, S, n, 0);
+  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, z, tmp, S, n, 1);
+  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, tmp, z, S, n, 2);
+  hipLaunchKernelGGL(k_dst_scale, g, dim3(BS), 0, st, z, n, h);
+  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, z, tmp, S, n, 0);
+  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, tmp, z, S, n, 1);
+  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, z, tmp, S, n, 2);
+  (void)hipMemcpyAsync(z, tmp, sizeof(double) * N3, hipMemcpyDeviceToDevice, st);
+}
+// PCG pieces: out += r . z ; p = z + beta p with beta from scal
+__global__ __launch_bounds__(BS) void k_dot(const double* __restrict__ a, const double* __restrict__ b, size_t N3,
+                                            double* __restrict__ out) {
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double v[1] = {p < N3 ? a[p] * b[p] : 0.0};
+  block_atomic_add<1>(v, out);
+}
+// U += alpha p, r -= alpha Ap, alpha = rz(slot-1) / pAp(slot); |r|^2 -> rr
+__global__ __launch_bounds__(BS) void k_pcg_update(double* __restrict__ U, double* __restrict__ r,
+                                                   const double* __restrict__ pv, const double* __restrict__ Ap,
+                                                   size_t N3, const double* __restrict__ scal, int slot,
+                                                   double* __restrict__ rr) {
+  const double pap = scal[4 * slot], rz = scal[4 * (slot - 1) + 2];
+  const double al = pap != 0.0 ? rz / pap : 0.0;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  double v[1] = {0.0};
+  if (p < N3) {
+    U[p] += al * pv[p];
+    const double rv = r[p] - al * Ap[p];
+    r[p] = rv;
+    v[0] = rv * rv;
+  }
+  block_atomic_add<1>(v, rr);
+}
+void launch_pcg_update(hipStream_t st, double* U, double* r, const double* p, const double* Ap, size_t N3,
+                       const double* scal, int slot, double* rr) {
+  hipLaunchKernelGGL(k_pcg_update, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, U, r, p, Ap, N3, scal, slot,
+                     rr);
+}
+__global__ void k_pcg_dir(const double* __restrict__ z, double* __restrict__ pv, size_t N3,
+                          const double* __restrict__ scal, int slot) {
+  const double rz = scal[4 * slot + 2], rzp = scal[4 * (slot - 1) + 2];
+  const double be = rzp != 0.0 ? rz / rzp : 0.0;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p < N3) pv[p] = z[p] + be * pv[p];
+}
+void launch_pcg_dir(hipStream_t st, const double* z, double* p, size_t N3, const double* scal, int slot) {
+  hipLaunchKernelGGL(k_pcg_dir, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, z, p, N3, scal, slot);
+}
+void launch_dot(hipStream_t st, const double* a, const double* b, size_t N3, double* out) {
+  hipLaunchKernelGGL(k_dot, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, a, b, N3, out);
 }
 
 // =============================================================== energies

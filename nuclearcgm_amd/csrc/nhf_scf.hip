@@ -10,6 +10,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <thread>
 #include <tuple>
 
 #include "nhf_internal.h"
@@ -48,6 +49,32 @@ struct GcgInfo {
   int basis = 0;
 };
 
+// GCG workspace of one species: its own stream, so that the neutron and
+// proton solves (and the Coulomb problem) run concurrently on the device
+// while two host threads do their small dense FP64 algebra.
+struct Work {
+  hipStream_t st = nullptr;
+  void *V, *V1, *AV, *invd;
+  double2 *gpart, *gout;
+  double *cg_scal, *thr, *rhs2, *lam_shift, *rn2, *norms, *scal_col;
+  int *done, *its, *clist;
+  // live stencil timing of the CG applications (optional)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  double st_ms = 0.0;
+  long st_launches = 0;
+  unsigned long long* act = nullptr;  // active column-launches (device counter)
+  // wall-time breakdown of the species step (seconds, accumulated)
+  double tph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+// phases: 0 fields (+ Coulomb), 1 guess orthonormalisation + first RR,
+// 2 block CG, 3 basis + Cholesky-QR, 4 projection + RR eigensolve + rotation,
+// 5 normalisation / Kramers order / conjugate direction
+using Clock = std::chrono::steady_clock;
+static inline double secs(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double>(b - a).count();
+}
+
 struct Problem {
   nhf_config cfg;
   int n = 0;
@@ -66,26 +93,28 @@ struct Problem {
   // states (precision of the run), double buffered, + conjugate directions
   void* psi[2][2];
   void* conj[2];
-  // GCG work
-  void *V, *V1, *AV, *invd;
-  double2 *gpart, *gout;
+  // GCG work, one per species
+  Work W[2];
   int gchunks = 0;
-  double *cg_scal, *thr, *rhs2, *lam_shift, *rn2, *norms, *scal_col;
-  int *done, *its;
   // Poisson
-  double *prhs, *pr, *pp, *pAp, *pscal;
+  double *prhs, *pr, *pp, *pAp, *pscal, *pz, *ptmp, *pS;
   int pmaxit = 0;
   // pairing, moments, energies
   double *occ[2], *mom, *en, *r2b, *sdens, *Gpair;
   int* dcols;
   // host state
   bool have = false;
+  double tcommon = 0.0, tenergy = 0.0;  // wall time of the shared update / energies
   BcsState bcs[2];
   std::vector<ConState> cons;
   double energyDiff = 1.0;
   double massCorr = MNUC;
   ~Problem() {
-    for (void* p : allocs) hipFree(p);
+    for (int q = 0; q < 2; ++q) {
+      for (hipEvent_t e : W[q].ev) (void)hipEventDestroy(e);
+      if (W[q].st) (void)hipStreamDestroy(W[q].st);
+    }
+    for (void* p : allocs) (void)hipFree(p);
   }
   template <typename T>
   int alloc(T*& p, size_t count) {
@@ -160,32 +189,48 @@ static int setup_problem(Problem& P) {
     CHK(P.alloc(tmp, spb * N3 * P.M[q]));
     P.conj[q] = tmp;
   }
-  CHK(P.alloc(tmp, spb * N3 * P.kmax));
-  P.V = tmp;
-  CHK(P.alloc(tmp, spb * N3 * P.kmax));
-  P.V1 = tmp;
-  CHK(P.alloc(tmp, spb * N3 * P.kmax));
-  P.AV = tmp;
-  CHK(P.alloc(tmp, rb * N3));
-  P.invd = tmp;
   P.gchunks = gram_chunks(N3);
-  CHK(P.alloc(P.gpart, (size_t)P.gchunks * P.kmax * P.kmax));
-  CHK(P.alloc(P.gout, (size_t)P.kmax * P.kmax));
-  CHK(P.alloc(P.cg_scal, (size_t)(P.steps_cap + 1) * P.Mmax * CG_NS));
-  CHK(P.alloc(P.thr, P.Mmax));
-  CHK(P.alloc(P.rhs2, P.Mmax));
-  CHK(P.alloc(P.lam_shift, P.kmax));
-  CHK(P.alloc(P.rn2, P.kmax + 8));  // the rotate kernel adds 8 columns per workgroup
-  CHK(P.alloc(P.norms, P.kmax));
-  CHK(P.alloc(P.scal_col, P.kmax));
-  CHK(P.alloc(P.done, P.Mmax));
-  CHK(P.alloc(P.its, P.Mmax));
+  for (int q = 0; q < 2; ++q) {
+    Work& W = P.W[q];
+    HIPCHK(hipStreamCreateWithFlags(&W.st, hipStreamNonBlocking));
+    CHK(P.alloc(tmp, spb * N3 * P.kmax));
+    W.V = tmp;
+    CHK(P.alloc(tmp, spb * N3 * P.kmax));
+    W.V1 = tmp;
+    CHK(P.alloc(tmp, spb * N3 * P.kmax));
+    W.AV = tmp;
+    CHK(P.alloc(tmp, rb * N3));
+    W.invd = tmp;
+    CHK(P.alloc(W.gpart, (size_t)P.gchunks * P.kmax * P.kmax));
+    CHK(P.alloc(W.gout, (size_t)P.kmax * P.kmax));
+    CHK(P.alloc(W.cg_scal, (size_t)(P.steps_cap + 1) * P.Mmax * CG_NS));
+    CHK(P.alloc(W.thr, P.Mmax));
+    CHK(P.alloc(W.rhs2, P.Mmax));
+    CHK(P.alloc(W.lam_shift, P.kmax));
+    CHK(P.alloc(W.rn2, P.kmax + 8));  // the rotate kernel adds 8 columns per workgroup
+    CHK(P.alloc(W.norms, P.kmax));
+    CHK(P.alloc(W.scal_col, P.kmax));
+    CHK(P.alloc(W.done, P.Mmax));
+    CHK(P.alloc(W.its, P.Mmax));
+    CHK(P.alloc(W.clist, P.Mmax));
+    CHK(P.alloc(W.act, 1));
+  }
   P.pmaxit = 20000;
   CHK(P.alloc(P.prhs, N3));
   CHK(P.alloc(P.pr, N3));
   CHK(P.alloc(P.pp, N3));
   CHK(P.alloc(P.pAp, N3));
-  CHK(P.alloc(P.pscal, 2 * (size_t)(P.pmaxit + 1)));
+  CHK(P.alloc(P.pscal, 4 * (size_t)(P.pmaxit + 1)));
+  CHK(P.alloc(P.pz, N3));
+  CHK(P.alloc(P.ptmp, N3));
+  CHK(P.alloc(P.pS, (size_t)c.n * c.n));
+  {
+    std::vector<double> S((size_t)c.n * c.n);
+    const double nrm = std::sqrt(2.0 / (c.n + 1));
+    for (int j = 0; j < c.n; ++j)
+      for (int k = 0; k < c.n; ++k) S[(size_t)j * c.n + k] = nrm * std::sin(PI_ * (j + 1) * (k + 1) / (c.n + 1));
+    HIPCHK(hipMemcpy(P.pS, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice));
+  }
   CHK(P.alloc(P.mom, NHF_MAX_CON + 2));
   CHK(P.alloc(P.en, 16));
   CHK(P.alloc(P.r2b, 6));
@@ -202,41 +247,43 @@ static int setup_problem(Problem& P) {
 
 // ------------------------------------------------------- small helpers
 template <typename R>
-static int gram_host(Problem& P, const Sp<R>* A, int ka, const Sp<R>* B, int kb, std::vector<cd>& out) {
-  launch_gram<R>(P.st, A, P.N3, ka, B, P.N3, kb, P.N3, P.gpart, P.gchunks, P.gout);
+static int gram_host(Problem& P, Work& W, const Sp<R>* A, int ka, const Sp<R>* B, int kb, std::vector<cd>& out) {
+  launch_gram<R>(W.st, A, P.N3, ka, B, P.N3, kb, P.N3, W.gpart, P.gchunks, W.gout, A == B ? 1 : 0);
   out.resize((size_t)ka * kb);
-  HIPCHK(hipMemcpyAsync(out.data(), P.gout, sizeof(double2) * ka * kb, hipMemcpyDeviceToHost, P.st));
-  HIPCHK(hipStreamSynchronize(P.st));
+  HIPCHK(hipMemcpyAsync(out.data(), W.gout, sizeof(double2) * ka * kb, hipMemcpyDeviceToHost, W.st));
+  HIPCHK(hipStreamSynchronize(W.st));
   return NHF_OK;
 }
 
 template <typename R>
-static int col_norms_host(Problem& P, const Sp<R>* X, int ncols, int trap, std::vector<double>& out) {
-  HIPCHK(hipMemsetAsync(P.norms, 0, sizeof(double) * ncols, P.st));
-  launch_col_norms<R>(P.st, X, P.N3, P.N3, ncols, P.n, trap, P.norms);
+static int col_norms_host(Problem& P, Work& W, const Sp<R>* X, int ncols, int trap, std::vector<double>& out) {
+  HIPCHK(hipMemsetAsync(W.norms, 0, sizeof(double) * ncols, W.st));
+  launch_col_norms<R>(W.st, X, P.N3, P.N3, ncols, P.n, trap, W.norms);
   out.resize(ncols);
-  HIPCHK(hipMemcpyAsync(out.data(), P.norms, sizeof(double) * ncols, hipMemcpyDeviceToHost, P.st));
-  HIPCHK(hipStreamSynchronize(P.st));
+  HIPCHK(hipMemcpyAsync(out.data(), W.norms, sizeof(double) * ncols, hipMemcpyDeviceToHost, W.st));
+  HIPCHK(hipStreamSynchronize(W.st));
   return NHF_OK;
 }
 
 template <typename R>
-static int scale_cols(Problem& P, Sp<R>* X, int ncols, const std::vector<double>& s) {
-  HIPCHK(hipMemcpyAsync(P.scal_col, s.data(), sizeof(double) * ncols, hipMemcpyHostToDevice, P.st));
-  launch_col_scale<R>(P.st, X, P.N3, P.N3, ncols, P.scal_col);
+static int scale_cols(Problem& P, Work& W, Sp<R>* X, int ncols, const std::vector<double>& s) {
+  HIPCHK(hipMemcpyAsync(W.scal_col, s.data(), sizeof(double) * ncols, hipMemcpyHostToDevice, W.st));
+  launch_col_scale<R>(W.st, X, P.N3, P.N3, ncols, W.scal_col);
+  HIPCHK(hipStreamSynchronize(W.st));  // the host vector must outlive the copy
   return NHF_OK;
 }
 
 template <typename R>
-static int rotate_dev(Problem& P, const Sp<R>* V, int k, const std::vector<cd>& C, int ncols, Sp<R>* Y,
+static int rotate_dev(Problem& P, Work& W, const Sp<R>* V, int k, const std::vector<cd>& C, int ncols, Sp<R>* Y,
                       const Sp<R>* AV, const std::vector<double>* w, bool accumulate) {
-  HIPCHK(hipMemcpyAsync(P.gout, C.data(), sizeof(double2) * (size_t)k * ncols, hipMemcpyHostToDevice, P.st));
+  HIPCHK(hipMemcpyAsync(W.gout, C.data(), sizeof(double2) * (size_t)k * ncols, hipMemcpyHostToDevice, W.st));
   if (AV) {
-    HIPCHK(hipMemcpyAsync(P.lam_shift, w->data(), sizeof(double) * ncols, hipMemcpyHostToDevice, P.st));
-    HIPCHK(hipMemsetAsync(P.rn2, 0, sizeof(double) * ncols, P.st));
+    HIPCHK(hipMemcpyAsync(W.lam_shift, w->data(), sizeof(double) * ncols, hipMemcpyHostToDevice, W.st));
+    HIPCHK(hipMemsetAsync(W.rn2, 0, sizeof(double) * (ncols + 8), W.st));
   }
-  launch_rotate<R>(P.st, V, P.N3, k, P.gout, ncols, Y, P.N3, P.N3, AV, AV ? P.lam_shift : nullptr,
-                   AV ? P.rn2 : nullptr, accumulate ? 1 : 0);
+  launch_rotate<R>(W.st, V, P.N3, k, W.gout, ncols, Y, P.N3, P.N3, AV, AV ? W.lam_shift : nullptr,
+                   AV ? W.rn2 : nullptr, accumulate ? 1 : 0);
+  HIPCHK(hipStreamSynchronize(W.st));  // the host coefficients must outlive the copy
   return NHF_OK;
 }
 
@@ -260,37 +307,40 @@ const float* fld_of<float>(Problem& P, int q) {
 // span[X | P | W] (W enters as W - X, which spans the same space and keeps
 // the Gram matrix well conditioned), residual locking, ascending order.
 template <typename R>
-static int gcg_dev(Problem& P, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const Sp<R>* conj, int pc,
+static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const Sp<R>* conj, int pc,
                    const nhf_gcg_params& prm, std::vector<double>& eps, GcgInfo* info) {
-  hipStream_t st = P.st;
+  hipStream_t st = Wk.st;
   const size_t N3 = P.N3, ld = N3;
   const int M = P.M[q];
   const R* fld = fld_of<R>(P, q);
-  Sp<R>* V = (Sp<R>*)P.V;
-  Sp<R>* V1 = (Sp<R>*)P.V1;
-  Sp<R>* AV = (Sp<R>*)P.AV;
-  R* invd = (R*)P.invd;
+  Sp<R>* V = (Sp<R>*)Wk.V;
+  Sp<R>* V1 = (Sp<R>*)Wk.V1;
+  Sp<R>* AV = (Sp<R>*)Wk.AV;
+  R* invd = (R*)Wk.invd;
   std::vector<cd> G, C1, C2, Pm;
   std::vector<int> keep;
+  auto tA = Clock::now();
   // 1. X = orthonormalised guess, into V1[0:M]
-  CHK(gram_host<R>(P, psi_in, M, psi_in, M, G));
+  CHK(gram_host<R>(P, Wk, psi_in, M, psi_in, M, G));
   int ke = cholqr_coeffs(M, G.data(), 1e-14, keep, C1);
   if (ke < M) return NHF_ESOLVER;
-  CHK(rotate_dev<R>(P, psi_in, M, C1, M, V1, nullptr, nullptr, false));
+  CHK(rotate_dev<R>(P, Wk, psi_in, M, C1, M, V1, nullptr, nullptr, false));
   // 2. Rayleigh-Ritz on X with shift 0
   launch_hpsi<R>(st, V1, AV, ld, M, fld, P.n, P.h, 0.0, nullptr, 0, nullptr);
-  CHK(gram_host<R>(P, V1, M, AV, M, Pm));
+  CHK(gram_host<R>(P, Wk, V1, M, AV, M, Pm));
   std::vector<double> w(M);
   std::vector<cd> Cx((size_t)M * M);
   if (!eigh_lower(M, Pm.data(), w.data(), Cx.data())) return NHF_ESOLVER;
-  CHK(rotate_dev<R>(P, V1, M, Cx, M, V, nullptr, nullptr, false));  // X -> V[0:M]
+  CHK(rotate_dev<R>(P, Wk, V1, M, Cx, M, V, nullptr, nullptr, false));  // X -> V[0:M]
   std::vector<double> Lam = w;
   double shift = 0.0;
   if (info) *info = GcgInfo();
+  auto tB = Clock::now();
+  Wk.tph[1] += secs(tA, tB);
   for (int pass = 0; pass < std::max(1, prm.max_iter); ++pass) {
     if (M > 1) shift = (Lam[M - 1] - 100.0 * Lam[0]) / 99.0;
     Sp<R>* X = V;
-    Sp<R>* W = V + (size_t)(M + pc) * ld;
+    Sp<R>* Wb = V + (size_t)(M + pc) * ld;
     Sp<R>* Rr = V1;
     Sp<R>* Pd = V1 + (size_t)M * ld;
     Sp<R>* T = V1 + (size_t)2 * M * ld;
@@ -299,63 +349,105 @@ static int gcg_dev(Problem& P, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const
     launch_hpsi<R>(st, X, T, ld, M, fld, P.n, P.h, shift, nullptr, 0, nullptr);
     std::vector<double> ls(M);
     for (int c = 0; c < M; ++c) ls[c] = Lam[c] + shift;
-    HIPCHK(hipMemcpyAsync(P.lam_shift, ls.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(Wk.lam_shift, ls.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
     const int steps = std::min(prm.steps, P.steps_cap);
-    HIPCHK(hipMemsetAsync(P.cg_scal, 0, sizeof(double) * (size_t)(steps + 1) * M * CG_NS, st));
-    HIPCHK(hipMemsetAsync(P.rhs2, 0, sizeof(double) * M, st));
-    HIPCHK(hipMemsetAsync(P.its, 0, sizeof(int) * M, st));
-    launch_cg_init<R>(st, X, P.lam_shift, W, Rr, Pd, T, invd, N3, ld, M, P.cg_scal, P.rhs2);
-    launch_cg_check0(st, P.cg_scal, P.rhs2, prm.cg_tol, M, P.thr, P.done);
-    std::vector<int> hdone(M);
-    for (int it = 0; it < steps; ++it) {
-      double* sprev = P.cg_scal + (size_t)it * M * CG_NS;
-      double* scur = P.cg_scal + (size_t)(it + 1) * M * CG_NS;
-      launch_hpsi<R>(st, Pd, T, ld, M, fld, P.n, P.h, shift, scur, CG_NS, P.done);
-      launch_cg_update<R>(st, W, Rr, Pd, T, invd, N3, ld, M, sprev, scur, P.done);
-      launch_cg_dir<R>(st, Rr, Pd, invd, N3, ld, M, sprev, scur, P.thr, P.done, P.its);
-      if ((it % 8) == 7 && it + 1 < steps) {
-        HIPCHK(hipMemcpyAsync(hdone.data(), P.done, sizeof(int) * M, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemsetAsync(Wk.cg_scal, 0, sizeof(double) * (size_t)(steps + 1) * M * CG_NS, st));
+    HIPCHK(hipMemsetAsync(Wk.rhs2, 0, sizeof(double) * M, st));
+    HIPCHK(hipMemsetAsync(Wk.its, 0, sizeof(int) * M, st));
+    launch_cg_init<R>(st, X, Wk.lam_shift, Wb, Rr, Pd, T, invd, N3, ld, M, Wk.cg_scal, Wk.rhs2);
+    launch_cg_check0(st, Wk.cg_scal, Wk.rhs2, prm.cg_tol, M, Wk.thr, Wk.done);
+    // active-column list: the CG kernels enumerate only the columns that
+    // were not yet converged at the last host check (every 4 iterations)
+    std::vector<int> hdone(M), act;
+    HIPCHK(hipMemcpyAsync(hdone.data(), Wk.done, sizeof(int) * M, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int c = 0; c < M; ++c)
+      if (!hdone[c]) act.push_back(c);
+    int ncl = (int)act.size();
+    if (ncl > 0) {
+      HIPCHK(hipMemcpyAsync(Wk.clist, act.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    int nev = 0;
+    if (Wk.timing) {
+      while ((int)Wk.ev.size() < 2 * steps) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        Wk.ev.push_back(e);
+      }
+    }
+    for (int it = 0; it < steps && ncl > 0; ++it) {
+      double* sprev = Wk.cg_scal + (size_t)it * M * CG_NS;
+      double* scur = Wk.cg_scal + (size_t)(it + 1) * M * CG_NS;
+      if (Wk.timing) HIPCHK(hipEventRecord(Wk.ev[2 * nev], st));
+      launch_hpsi<R>(st, Pd, T, ld, ncl, fld, P.n, P.h, shift, scur, CG_NS, Wk.done, Wk.timing ? Wk.act : nullptr,
+                     Wk.clist);
+      if (Wk.timing) HIPCHK(hipEventRecord(Wk.ev[2 * nev++ + 1], st));
+      launch_cg_update<R>(st, Wb, Rr, Pd, T, invd, N3, ld, ncl, sprev, scur, Wk.done, Wk.clist);
+      launch_cg_dir<R>(st, Rr, Pd, invd, N3, ld, ncl, sprev, scur, Wk.thr, Wk.done, Wk.its, Wk.clist);
+      if ((it % 4) == 3 && it + 1 < steps) {
+        HIPCHK(hipMemcpyAsync(hdone.data(), Wk.done, sizeof(int) * M, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        bool all = true;
-        for (int c = 0; c < M; ++c) all = all && hdone[c];
-        if (all) break;
+        act.clear();
+        for (int c = 0; c < M; ++c)
+          if (!hdone[c]) act.push_back(c);
+        if ((int)act.size() < ncl && !act.empty()) {
+          HIPCHK(hipMemcpyAsync(Wk.clist, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, st));
+          HIPCHK(hipStreamSynchronize(st));
+        }
+        ncl = (int)act.size();
       }
     }
     if (info) {
       std::vector<int> its(M);
-      HIPCHK(hipMemcpyAsync(its.data(), P.its, sizeof(int) * M, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(its.data(), Wk.its, sizeof(int) * M, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       for (int c = 0; c < M; ++c) info->cg_iters += its[c];
     }
+    if (Wk.timing) {
+      HIPCHK(hipStreamSynchronize(st));
+      for (int e = 0; e < nev; ++e) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, Wk.ev[2 * e], Wk.ev[2 * e + 1]));
+        Wk.st_ms += ms;
+      }
+      Wk.st_launches += nev;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    auto tC = Clock::now();
+    Wk.tph[2] += secs(tB, tC);
     // 4. V = [X | P | W - X], unit columns
     if (pc > 0)
       HIPCHK(hipMemcpyAsync(V + (size_t)M * ld, conj, sizeof(Sp<R>) * ld * pc, hipMemcpyDeviceToDevice, st));
     {
       double am = -1.0, bp = 1.0;
-      launch_axpby_cols<R>(st, X, W, N3, ld, M, &am, &bp);
+      launch_axpby_cols<R>(st, X, Wb, N3, ld, M, &am, &bp);
     }
     const int k = 2 * M + pc;
-    std::vector<double> nr;
-    CHK(col_norms_host<R>(P, V, k, 0, nr));
-    std::vector<double> sc(k);
-    for (int c = 0; c < k; ++c) sc[c] = nr[c] > 0.0 ? 1.0 / std::sqrt(nr[c]) : 0.0;
-    CHK(scale_cols<R>(P, V, k, sc));
-    // 5. Cholesky-QR twice with column dropping
-    CHK(gram_host<R>(P, V, k, V, k, G));
-    int k1 = cholqr_coeffs(k, G.data(), 1e-14, keep, C1);
+    launch_col_normalize<R>(st, V, N3, ld, k, P.n, 0, 1.0, Wk.norms);
+    // 5. Cholesky-QR with column dropping; a second pass only when the basis
+    //    was ill conditioned (smallest new-direction norm^2 below 1e-8)
+    CHK(gram_host<R>(P, Wk, V, k, V, k, G));
+    double mrel = 1.0;
+    int k1 = cholqr_coeffs(k, G.data(), 1e-14, keep, C1, &mrel);
     if (k1 < 1) return NHF_ESOLVER;
-    CHK(rotate_dev<R>(P, V, k, C1, k1, V1, nullptr, nullptr, false));
-    CHK(gram_host<R>(P, V1, k1, V1, k1, G));
-    // C2: k1 x k2 (columns that lost rank in the first pass drop out here)
-    int k2 = cholqr_coeffs(k1, G.data(), 1e-14, keep, C2);
-    if (k2 < 1) return NHF_ESOLVER;
+    CHK(rotate_dev<R>(P, Wk, V, k, C1, k1, V1, nullptr, nullptr, false));
+    int k2 = k1;
+    bool pass2 = mrel < 1e-8;
+    if (pass2) {
+      CHK(gram_host<R>(P, Wk, V1, k1, V1, k1, G));
+      k2 = cholqr_coeffs(k1, G.data(), 1e-14, keep, C2);  // k1 x k2
+      if (k2 < 1) return NHF_ESOLVER;
+    }
     if (info) info->basis = k2;
+    auto tD = Clock::now();
+    Wk.tph[3] += secs(tC, tD);
     // 6. Rayleigh-Ritz: Pm = C2^H (V1^H (H + sigma) V1) C2
     launch_hpsi<R>(st, V1, AV, ld, k1, fld, P.n, P.h, shift, nullptr, 0, nullptr);
     std::vector<cd> P1;
-    CHK(gram_host<R>(P, V1, k1, AV, k1, P1));
-    Pm.assign((size_t)k2 * k2, 0.0);
-    {
+    CHK(gram_host<R>(P, Wk, V1, k1, AV, k1, P1));
+    if (pass2) {
+      Pm.assign((size_t)k2 * k2, 0.0);
       std::vector<cd> T1((size_t)k1 * k2, 0.0);  // P1 C2
       for (int c = 0; c < k2; ++c)
         for (int l = 0; l < k1; ++l) {
@@ -369,22 +461,28 @@ static int gcg_dev(Problem& P, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const
           for (int l = 0; l < k1; ++l) s += std::conj(C2[(size_t)r * k1 + l]) * T1[(size_t)c * k1 + l];
           Pm[(size_t)c * k2 + r] = s;
         }
+    } else {
+      Pm.swap(P1);
     }
     std::vector<double> wk(k2);
     std::vector<cd> Ck((size_t)k2 * k2);
     if (!eigh_lower(k2, Pm.data(), wk.data(), Ck.data())) return NHF_ESOLVER;
     const int rr = std::min(M, k2);
     std::vector<cd> Ctot((size_t)k1 * rr, 0.0);
-    for (int c = 0; c < rr; ++c)
-      for (int l = 0; l < k2; ++l) {
-        cd cf = Ck[(size_t)c * k2 + l];
-        for (int r = 0; r < k1; ++r) Ctot[(size_t)c * k1 + r] += C2[(size_t)l * k1 + r] * cf;
-      }
+    if (pass2) {
+      for (int c = 0; c < rr; ++c)
+        for (int l = 0; l < k2; ++l) {
+          cd cf = Ck[(size_t)c * k2 + l];
+          for (int r = 0; r < k1; ++r) Ctot[(size_t)c * k1 + r] += C2[(size_t)l * k1 + r] * cf;
+        }
+    } else {
+      std::copy(Ck.begin(), Ck.begin() + (size_t)k1 * rr, Ctot.begin());
+    }
     std::vector<double> wr(wk.begin(), wk.begin() + rr);
     Sp<R>* Xout = (pass + 1 == std::max(1, prm.max_iter)) ? psi_out : V;
-    CHK(rotate_dev<R>(P, V1, k1, Ctot, rr, Xout, AV, &wr, false));
+    CHK(rotate_dev<R>(P, Wk, V1, k1, Ctot, rr, Xout, AV, &wr, false));
     std::vector<double> rn2(rr);
-    HIPCHK(hipMemcpyAsync(rn2.data(), P.rn2, sizeof(double) * rr, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(rn2.data(), Wk.rn2, sizeof(double) * rr, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     // locking: converged columns report the Rayleigh quotient Re(x^H A x)
     Lam.assign(M, 0.0);
@@ -402,8 +500,9 @@ static int gcg_dev(Problem& P, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const
       }
       Lam[c] = lam;
     }
-    if (rr < M)
-      HIPCHK(hipMemsetAsync(Xout + (size_t)rr * ld, 0, sizeof(Sp<R>) * ld * (M - rr), st));
+    if (rr < M) HIPCHK(hipMemsetAsync(Xout + (size_t)rr * ld, 0, sizeof(Sp<R>) * ld * (M - rr), st));
+    Wk.tph[4] += secs(tD, Clock::now());
+    tB = Clock::now();
     if (Xout == psi_out) break;
   }
   // ascending order of the final values
@@ -418,6 +517,7 @@ static int gcg_dev(Problem& P, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const
       HIPCHK(hipMemcpyAsync(psi_out + (size_t)c * ld, V + (size_t)ord[c] * ld, sizeof(Sp<R>) * ld,
                             hipMemcpyDeviceToDevice, st));
   }
+  HIPCHK(hipStreamSynchronize(st));
   eps.resize(M);
   for (int c = 0; c < M; ++c) eps[c] = Lam[ord[c]];
   return NHF_OK;
@@ -426,29 +526,36 @@ static int gcg_dev(Problem& P, int q, const Sp<R>* psi_in, Sp<R>* psi_out, const
 // ------------------------------------------------------------- Coulomb
 // CG on the SPD operator -lap4 with the ghost-shell right-hand side; the
 // moments (Q20, Q22) of the proton density must be in P.mom.
-static int poisson_dev(Problem& P, const double* rho_p, double* U, double tol, int maxit, int* iters) {
-  hipStream_t st = P.st;
+static int poisson_dev(Problem& P, hipStream_t st, const double* rho_p, double* U, double tol, int maxit,
+                       int* iters) {
+  // preconditioned CG (DST preconditioner): scal[4 it + {0: pAp, 1: rr, 2: rz}]
   const size_t N3 = P.N3;
   maxit = std::min(maxit, P.pmaxit);
   launch_coulomb_rhs(st, P.n, P.a, P.h, P.cfg.Z, rho_p, P.mom, P.prhs);
   launch_lap4_dot(st, U, P.pAp, P.n, P.h, nullptr);
-  HIPCHK(hipMemsetAsync(P.pscal, 0, sizeof(double) * 2 * (size_t)(maxit + 1), st));
+  HIPCHK(hipMemsetAsync(P.pscal, 0, sizeof(double) * 4 * (size_t)(maxit + 1), st));
+  // r = b - A U, |b|^2 -> scal[0], |r|^2 -> scal[1]; then z = M^-1 r, p = z
   launch_poisson_init(st, P.prhs, P.pAp, P.pr, P.pp, N3, P.pscal);
-  double s0[2];
-  HIPCHK(hipMemcpyAsync(s0, P.pscal, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+  launch_dst_precond(st, P.pr, P.pz, P.ptmp, P.pS, P.n, P.h);
+  launch_dot(st, P.pr, P.pz, N3, P.pscal + 2);
+  HIPCHK(hipMemcpyAsync(P.pp, P.pz, sizeof(double) * N3, hipMemcpyDeviceToDevice, st));
+  double s0[3];
+  HIPCHK(hipMemcpyAsync(s0, P.pscal, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   const double bn = s0[0];
-  // slot 0 holds {|b|^2, rr0}; the update kernel reads rr of slot-1 from [1]
   const double thr = tol * tol * bn;
   int it = 0;
   if (s0[1] > thr) {
     for (it = 1; it <= maxit; ++it) {
-      launch_lap4_dot(st, P.pp, P.pAp, P.n, P.h, P.pscal + 2 * it);
-      launch_poisson_update(st, U, P.pr, P.pp, P.pAp, N3, P.pscal, it, P.pscal + 2 * it + 1);
-      launch_poisson_dir(st, P.pr, P.pp, N3, P.pscal, it);
-      if (it % 16 == 0 || it == maxit) {
+      double* sc = P.pscal + 4 * it;
+      launch_lap4_dot(st, P.pp, P.pAp, P.n, P.h, sc);                   // pAp
+      launch_pcg_update(st, U, P.pr, P.pp, P.pAp, N3, P.pscal, it, sc + 1);  // alpha = rz_prev / pAp; rr
+      launch_dst_precond(st, P.pr, P.pz, P.ptmp, P.pS, P.n, P.h);
+      launch_dot(st, P.pr, P.pz, N3, sc + 2);                          // rz
+      launch_pcg_dir(st, P.pz, P.pp, N3, P.pscal, it);                  // beta = rz / rz_prev
+      if (it % 4 == 0 || it == maxit) {
         double rr;
-        HIPCHK(hipMemcpyAsync(&rr, P.pscal + 2 * it + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&rr, sc + 1, sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (rr <= thr) break;
       }
@@ -496,13 +603,13 @@ static double constraint_coef(Problem& P, ConState& k, double Q) {
 }
 
 template <typename R>
-static int bcs_species(Problem& P, int q, const Sp<R>* psi, const std::vector<double>& eps, BcsState& out) {
+static int bcs_species(Problem& P, Work& W, int q, const Sp<R>* psi, const std::vector<double>& eps, BcsState& out) {
   const int M = P.M[q];
   const size_t N3 = P.N3;
-  Sp<R>* T = (Sp<R>*)P.V1;
-  launch_time_reverse<R>(P.st, psi, T, N3 * M);
+  Sp<R>* T = (Sp<R>*)W.V1;
+  launch_time_reverse<R>(W.st, psi, T, N3 * M);
   std::vector<cd> O;
-  CHK(gram_host<R>(P, psi, M, T, M, O));
+  CHK(gram_host<R>(P, W, psi, M, T, M, O));
   std::vector<int> partner = match_time_reversal(O, M);
   std::vector<int> uniq, s2p;
   std::vector<char> vis;
@@ -511,12 +618,12 @@ static int bcs_species(Problem& P, int q, const Sp<R>* psi, const std::vector<do
   std::vector<double> ep(Pn), G((size_t)Pn * Pn, 0.0);
   for (int p = 0; p < Pn; ++p) ep[p] = eps[uniq[p]];
   if (Pn > 0) {
-    HIPCHK(hipMemcpyAsync(P.dcols, uniq.data(), sizeof(int) * Pn, hipMemcpyHostToDevice, P.st));
-    launch_state_density<R>(P.st, psi, N3, P.dcols, Pn, N3, P.sdens);
-    HIPCHK(hipMemsetAsync(P.Gpair, 0, sizeof(double) * Pn * Pn, P.st));
-    launch_pair_matrix(P.st, P.sdens, Pn, P.n, P.h, P.Gpair);
-    HIPCHK(hipMemcpyAsync(G.data(), P.Gpair, sizeof(double) * Pn * Pn, hipMemcpyDeviceToHost, P.st));
-    HIPCHK(hipStreamSynchronize(P.st));
+    HIPCHK(hipMemcpyAsync(P.dcols, uniq.data(), sizeof(int) * Pn, hipMemcpyHostToDevice, W.st));
+    launch_state_density<R>(W.st, psi, N3, P.dcols, Pn, N3, P.sdens);
+    HIPCHK(hipMemsetAsync(P.Gpair, 0, sizeof(double) * Pn * Pn, W.st));
+    launch_pair_matrix(W.st, P.sdens, Pn, P.n, P.h, P.Gpair);
+    HIPCHK(hipMemcpyAsync(G.data(), P.Gpair, sizeof(double) * Pn * Pn, hipMemcpyDeviceToHost, W.st));
+    HIPCHK(hipStreamSynchronize(W.st));
   }
   const double V0 = P.pairing(q).V0;
   std::vector<double> Gs((size_t)Pn * Pn);
@@ -558,14 +665,15 @@ static int bcs_species(Problem& P, int q, const Sp<R>* psi, const std::vector<do
   return NHF_OK;
 }
 
-// IterationData::updateQuantities (iteration_data.cpp:852-1125) for HF and
-// HF+BCS: occupations, densities with linear mixing, Coulomb, constraint
-// fields, single-particle fields.
+// IterationData::updateQuantities (iteration_data.cpp:852-1125), common
+// part: occupations (solvePairingBCS, :384-465), densities with linear
+// mixing, density moments and the constraint field coefficients.  The
+// per-species fields (and the Coulomb problem) follow in field_species.
 template <typename R>
-static int update_dev(Problem& P, Sp<R>* const psi[2], const std::vector<double> eps[2], int iter) {
+static int update_common(Problem& P, Sp<R>* const psi[2], const std::vector<double> eps[2], int iter,
+                         FieldParams& F) {
   hipStream_t st = P.st;
   const nhf_config& c = P.cfg;
-  // occupations (solvePairingBCS, iteration_data.cpp:384-465)
   BcsState nul[2];
   for (int q = 0; q < 2; ++q) {
     const int M = P.M[q];
@@ -589,7 +697,7 @@ static int update_dev(Problem& P, Sp<R>* const psi[2], const std::vector<double>
         continue;
       }
       BcsState b;
-      CHK(bcs_species<R>(P, q, psi[q], eps[q], b));
+      CHK(bcs_species<R>(P, P.W[q], q, psi[q], eps[q], b));
       P.bcs[q] = b;
     }
     double sn = 0, sp = 0;
@@ -602,12 +710,10 @@ static int update_dev(Problem& P, Sp<R>* const psi[2], const std::vector<double>
   }
   for (int q = 0; q < 2; ++q)
     HIPCHK(hipMemcpyAsync(P.occ[q], P.bcs[q].v2.data(), sizeof(double) * P.M[q], hipMemcpyHostToDevice, st));
-  // densities, mixed
   const int first = P.have ? 0 : 1;
   for (int q = 0; q < 2; ++q)
     launch_densities<R>(st, psi[q], P.N3, P.M[q], P.occ[q], P.n, P.h, c.density_mix, first, P.rho[q], P.tau[q],
                         P.J9[q], P.divJ[q]);
-  // moments for constraints and the Coulomb boundary
   int kinds[NHF_MAX_CON];
   for (size_t i = 0; i < P.cons.size(); ++i) kinds[i] = P.cons[i].c.kind;
   HIPCHK(hipMemsetAsync(P.mom, 0, sizeof(double) * (NHF_MAX_CON + 2), st));
@@ -615,7 +721,7 @@ static int update_dev(Problem& P, Sp<R>* const psi[2], const std::vector<double>
   double mom[NHF_MAX_CON + 2];
   HIPCHK(hipMemcpyAsync(mom, P.mom, sizeof(mom), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  FieldParams F = field_params(P);
+  F = field_params(P);
   if (iter > 0) {
     F.ncon = (int)P.cons.size();
     for (int i = 0; i < F.ncon; ++i) {
@@ -623,9 +729,16 @@ static int update_dev(Problem& P, Sp<R>* const psi[2], const std::vector<double>
       F.con_coef[i] = constraint_coef(P, P.cons[i], mom[i]);
     }
   }
-  if (c.coulomb) {
+  P.have = true;
+  return NHF_OK;
+}
+
+// fields of species q on its own stream (protons: Coulomb first)
+static int field_species(Problem& P, int q, const FieldParams& F) {
+  hipStream_t st = P.W[q].st;
+  if (q == 1 && P.cfg.coulomb) {
     int its = 0;
-    CHK(poisson_dev(P, P.rho[1], P.UCdir, 1e-13, P.pmaxit, &its));
+    CHK(poisson_dev(P, st, P.rho[1], P.UCdir, 1e-13, P.pmaxit, &its));
   }
   const double* rho[2] = {P.rho[0], P.rho[1]};
   const double* tau[2] = {P.tau[0], P.tau[1]};
@@ -633,9 +746,8 @@ static int update_dev(Problem& P, Sp<R>* const psi[2], const std::vector<double>
   const double* dj[2] = {P.divJ[0], P.divJ[1]};
   float* f32[2] = {P.fld32[0], P.fld32[1]};
   launch_fields(st, F, P.n, P.a, P.h, rho, tau, J9, dj, P.UCdir, P.grho, P.lrho, P.Jv, P.UCoul, P.fld64,
-                P.prec == NHF_PREC_FP64 ? nullptr : f32);
-  P.have = true;
-  P.fields_set[0] = P.fields_set[1] = true;
+                P.prec == NHF_PREC_FP64 ? nullptr : f32, 1 << q);
+  P.fields_set[q] = true;
   return NHF_OK;
 }
 
@@ -692,13 +804,13 @@ static double constraint_error(const ConState& k) {
 
 // Wavefunction::TROrder (wavefunction.cpp:581-636): Kramers partners adjacent
 template <typename R>
-static int tr_order_dev(Problem& P, int q, Sp<R>* psi) {
+static int tr_order_dev(Problem& P, Work& W, int q, Sp<R>* psi) {
   const int M = P.M[q];
   const size_t N3 = P.N3;
-  Sp<R>* T = (Sp<R>*)P.V1;
-  launch_time_reverse<R>(P.st, psi, T, N3 * M);
+  Sp<R>* T = (Sp<R>*)W.V1;
+  launch_time_reverse<R>(W.st, psi, T, N3 * M);
   std::vector<cd> O;
-  CHK(gram_host<R>(P, psi, M, T, M, O));  // O[j*M + i] = <psi_i | T psi_j>
+  CHK(gram_host<R>(P, W, psi, M, T, M, O));  // O[j*M + i] = <psi_i | T psi_j>
   std::vector<int> order;
   std::vector<bool> used(M, false);
   for (int i = 0; i < M && (int)order.size() < M; ++i) {
@@ -722,11 +834,48 @@ static int tr_order_dev(Problem& P, int q, Sp<R>* psi) {
   bool ident = true;
   for (int c = 0; c < (int)order.size(); ++c) ident = ident && order[c] == c;
   if (ident || (int)order.size() != M) return NHF_OK;
-  Sp<R>* tmp = (Sp<R>*)P.V;
-  HIPCHK(hipMemcpyAsync(tmp, psi, sizeof(Sp<R>) * N3 * M, hipMemcpyDeviceToDevice, P.st));
+  Sp<R>* tmp = (Sp<R>*)W.V;
+  HIPCHK(hipMemcpyAsync(tmp, psi, sizeof(Sp<R>) * N3 * M, hipMemcpyDeviceToDevice, W.st));
   for (int c = 0; c < M; ++c)
     HIPCHK(hipMemcpyAsync(psi + (size_t)c * N3, tmp + (size_t)order[c] * N3, sizeof(Sp<R>) * N3,
-                          hipMemcpyDeviceToDevice, P.st));
+                          hipMemcpyDeviceToDevice, W.st));
+  HIPCHK(hipStreamSynchronize(W.st));
+  return NHF_OK;
+}
+
+// Everything of one species between the density update and the energies:
+// its fields (Coulomb for protons), the GCG step, trapezoid normalisation
+// (Wavefunction::normalize), Kramers ordering and the conjugate direction
+// P = psi_new - psi_old (psi_old^H psi_new) with unit columns (main.cpp:
+// 229-282).  Runs on the species' own stream and host thread.
+template <typename R>
+static int species_step(Problem& P, int q, const FieldParams& F, hipEvent_t dens_ready, Sp<R>* pcur, Sp<R>* pnew,
+                        int it, int& pc, std::vector<double>& L) {
+  Work& W = P.W[q];
+  const nhf_config& c = P.cfg;
+  const int M = P.M[q];
+  const size_t N3 = P.N3;
+  HIPCHK(hipStreamWaitEvent(W.st, dens_ready, 0));
+  auto t0 = Clock::now();
+  CHK(field_species(P, q, F));
+  HIPCHK(hipStreamSynchronize(W.st));
+  W.tph[0] += secs(t0, Clock::now());
+  CHK(gcg_dev<R>(P, W, q, pcur, pnew, (const Sp<R>*)P.conj[q], pc, c.gcg, L, nullptr));
+  auto t5 = Clock::now();
+  launch_col_normalize<R>(W.st, pnew, N3, N3, M, P.n, 1, P.h * P.h * P.h, W.norms);
+  if (c.pairing_type != NHF_PAIR_NONE) CHK(tr_order_dev<R>(P, W, q, pnew));
+  if (it > 0) {
+    std::vector<cd> tmp;
+    CHK(gram_host<R>(P, W, pcur, M, pnew, M, tmp));
+    for (auto& v : tmp) v = -v;
+    Sp<R>* cj = (Sp<R>*)P.conj[q];
+    HIPCHK(hipMemcpyAsync(cj, pnew, sizeof(Sp<R>) * N3 * M, hipMemcpyDeviceToDevice, W.st));
+    CHK(rotate_dev<R>(P, W, pcur, M, tmp, M, cj, nullptr, nullptr, true));
+    launch_col_normalize<R>(W.st, cj, N3, N3, M, P.n, 0, 1.0, W.norms);
+    pc = M;
+  }
+  HIPCHK(hipStreamSynchronize(W.st));
+  W.tph[5] += secs(t5, Clock::now());
   return NHF_OK;
 }
 
@@ -737,61 +886,62 @@ static int scf_dev(Problem& P, std::vector<double> eps[2], nhf_result* res, doub
   const nhf_config& c = P.cfg;
   const size_t N3 = P.N3;
   const int N = c.A - c.Z, Z = c.Z;
+  const bool both = c.coulomb || N != Z;
   int cur = 0, pc[2] = {0, 0};
   double Eint_prev = 0.0, Ehf_prev = 0.0, T[NHF_NTERMS];
   std::fill(T, T + NHF_NTERMS, 0.0);
   bool conv = false;
   double tstop = -1.0;
+  hipEvent_t dens_ready;
+  HIPCHK(hipEventCreateWithFlags(&dens_ready, hipEventDisableTiming));
   auto t0 = std::chrono::steady_clock::now();
-  int it;
+  int it, rc = NHF_OK;
   for (it = 0; it < c.max_iter; ++it) {
     Sp<R>* pcur[2] = {(Sp<R>*)P.psi[0][cur], (Sp<R>*)P.psi[1][cur]};
     Sp<R>* pnew[2] = {(Sp<R>*)P.psi[0][cur ^ 1], (Sp<R>*)P.psi[1][cur ^ 1]};
-    CHK(update_dev<R>(P, pcur, eps, it));
-    std::vector<double> L[2];
-    for (int q = 0; q < 2; ++q) {
-      if (q == 1 && !(c.coulomb || N != Z)) {
-        HIPCHK(hipMemcpyAsync(pnew[1], pnew[0], sizeof(Sp<R>) * N3 * P.M[1], hipMemcpyDeviceToDevice, st));
-        L[1].assign(L[0].begin(), L[0].begin() + P.M[1]);
-        continue;
-      }
-      CHK(gcg_dev<R>(P, q, pcur[q], pnew[q], (const Sp<R>*)P.conj[q], pc[q], c.gcg, L[q], nullptr));
+    FieldParams F;
+    auto tu = Clock::now();
+    rc = update_common<R>(P, pcur, eps, it, F);
+    P.tcommon += secs(tu, Clock::now());
+    if (rc != NHF_OK) break;
+    if (hipEventRecord(dens_ready, st) != hipSuccess) {
+      rc = NHF_EHIP;
+      break;
     }
+    std::vector<double> L[2];
+    int rcq[2] = {NHF_OK, NHF_OK};
+    if (both) {
+      std::thread tn([&] { rcq[0] = species_step<R>(P, 0, F, dens_ready, pcur[0], pnew[0], it, pc[0], L[0]); });
+      rcq[1] = species_step<R>(P, 1, F, dens_ready, pcur[1], pnew[1], it, pc[1], L[1]);
+      tn.join();
+    } else {
+      rcq[0] = species_step<R>(P, 0, F, dens_ready, pcur[0], pnew[0], it, pc[0], L[0]);
+      if (rcq[0] == NHF_OK) {
+        // N = Z without Coulomb: protons are the first M_p neutron states
+        // (main.cpp:239-244); the proton conjugate directions likewise
+        Work& W = P.W[0];
+        if (hipMemcpyAsync(pnew[1], pnew[0], sizeof(Sp<R>) * N3 * P.M[1], hipMemcpyDeviceToDevice, W.st) !=
+                hipSuccess ||
+            hipMemcpyAsync(P.conj[1], P.conj[0], sizeof(Sp<R>) * N3 * P.M[1], hipMemcpyDeviceToDevice, W.st) !=
+                hipSuccess ||
+            hipStreamSynchronize(W.st) != hipSuccess)
+          rcq[1] = NHF_EHIP;
+        L[1].assign(L[0].begin(), L[0].begin() + P.M[1]);
+        pc[1] = pc[0] > 0 ? P.M[1] : 0;
+      }
+    }
+    rc = rcq[0] != NHF_OK ? rcq[0] : rcq[1];
+    if (rc != NHF_OK) break;
     double mx = 0.0;
     for (int q = 0; q < 2; ++q)
       for (int i = 0; i < P.M[q]; ++i) mx = std::max(mx, std::abs(L[q][i] - eps[q][i]));
-    // normalise with the trapezoid rule (Wavefunction::normalize)
-    for (int q = 0; q < 2; ++q) {
-      std::vector<double> nr;
-      CHK(col_norms_host<R>(P, pnew[q], P.M[q], 1, nr));
-      std::vector<double> sc(P.M[q]);
-      for (int i = 0; i < P.M[q]; ++i) sc[i] = 1.0 / std::sqrt(nr[i] * P.h * P.h * P.h);
-      CHK(scale_cols<R>(P, pnew[q], P.M[q], sc));
-    }
-    if (c.pairing_type != NHF_PAIR_NONE)
-      for (int q = 0; q < 2; ++q) CHK(tr_order_dev<R>(P, q, pnew[q]));
-    if (it > 0) {
-      // P = psi_new - psi_old (psi_old^H psi_new), unit columns
-      for (int q = 0; q < 2; ++q) {
-        const int M = P.M[q];
-        std::vector<cd> tmp;
-        CHK(gram_host<R>(P, pcur[q], M, pnew[q], M, tmp));
-        for (auto& v : tmp) v = -v;
-        Sp<R>* cj = (Sp<R>*)P.conj[q];
-        HIPCHK(hipMemcpyAsync(cj, pnew[q], sizeof(Sp<R>) * N3 * M, hipMemcpyDeviceToDevice, st));
-        CHK(rotate_dev<R>(P, pcur[q], M, tmp, M, cj, nullptr, nullptr, true));
-        std::vector<double> nr;
-        CHK(col_norms_host<R>(P, cj, M, 0, nr));
-        std::vector<double> sc(M);
-        for (int i = 0; i < M; ++i) sc[i] = nr[i] > 0.0 ? 1.0 / std::sqrt(nr[i]) : 0.0;
-        CHK(scale_cols<R>(P, cj, M, sc));
-        pc[q] = M;
-      }
-    }
     cur ^= 1;
     eps[0] = L[0];
     eps[1] = L[1];
-    CHK(energies_dev(P, T));
+    auto te = Clock::now();
+    rc = energies_dev(P, T);
+    P.tenergy += secs(te, Clock::now());
+    if (rc != NHF_OK) break;
     const double Eint = T[NHF_E_INT], Ehf = T[NHF_E_HF];
     bool cc = true;
     for (const auto& k : P.cons) {
@@ -815,6 +965,8 @@ static int scf_dev(Problem& P, std::vector<double> eps[2], nhf_result* res, doub
     Eint_prev = Eint;
     Ehf_prev = Ehf;
   }
+  (void)hipEventDestroy(dens_ready);
+  if (rc != NHF_OK) return rc;
   res->iterations = it;
   res->converged = conv ? 1 : 0;
   std::copy(T, T + NHF_NTERMS, res->terms);
@@ -839,13 +991,15 @@ static int scf_dev(Problem& P, std::vector<double> eps[2], nhf_result* res, doub
   return NHF_OK;
 }
 
-// host complex128 (interleaved) <-> device states of the run precision
+// host complex128 (interleaved) <-> device states of the run precision;
+// staging through the neutron workspace
 static int upload_states(Problem& P, const double* h, void* dst, int m) {
   const size_t cnt = P.N3 * (size_t)m;
   if (P.prec == NHF_PREC_FP64) {
     HIPCHK(hipMemcpyAsync(dst, h, sizeof(Sp<double>) * cnt, hipMemcpyHostToDevice, P.st));
+    HIPCHK(hipStreamSynchronize(P.st));
   } else {
-    Sp<double>* tmp = (Sp<double>*)P.AV;  // AV holds >= 3 Mmax FP32 columns = 1.5 Mmax FP64 columns
+    Sp<double>* tmp = (Sp<double>*)P.W[0].AV;  // AV holds >= 3 Mmax FP32 columns = 1.5 Mmax FP64 columns
     const size_t chunk = (size_t)P.N3 * (P.kmax / 2);
     for (size_t off = 0; off < cnt; off += chunk) {
       size_t c = std::min(chunk, cnt - off);
@@ -862,7 +1016,7 @@ static int download_states(Problem& P, const void* src, double* h, int m) {
     HIPCHK(hipMemcpyAsync(h, src, sizeof(Sp<double>) * cnt, hipMemcpyDeviceToHost, P.st));
     HIPCHK(hipStreamSynchronize(P.st));
   } else {
-    Sp<double>* tmp = (Sp<double>*)P.AV;
+    Sp<double>* tmp = (Sp<double>*)P.W[0].AV;
     const size_t chunk = (size_t)P.N3 * (P.kmax / 2);
     for (size_t off = 0; off < cnt; off += chunk) {
       size_t c = std::min(chunk, cnt - off);
@@ -1006,7 +1160,7 @@ int nhf_densities(void* ctx, int pid, const double* psi, int m, const double* oc
   Problem* P = get_problem(ctx, pid);
   if (!P || !psi || !occ || !rho || !tau || !J9 || !divJ || m < 1 || m > P->Mmax) return NHF_EINVAL;
   const size_t N3 = P->N3;
-  CHK(upload_states(*P, psi, P->V, m));
+  CHK(upload_states(*P, psi, P->W[0].V, m));
   HIPCHK(hipMemcpyAsync(P->occ[0], occ, sizeof(double) * m, hipMemcpyHostToDevice, P->st));
   double *r = nullptr, *t, *j, *d;
   if (hipMalloc(&r, sizeof(double) * 12 * N3) != hipSuccess) return NHF_ENOMEM;
@@ -1014,9 +1168,9 @@ int nhf_densities(void* ctx, int pid, const double* psi, int m, const double* oc
   j = t + N3;
   d = j + 9 * N3;
   if (P->prec == NHF_PREC_FP64)
-    launch_densities<double>(P->st, (const Sp<double>*)P->V, N3, m, P->occ[0], P->n, P->h, 1.0, 1, r, t, j, d);
+    launch_densities<double>(P->st, (const Sp<double>*)P->W[0].V, N3, m, P->occ[0], P->n, P->h, 1.0, 1, r, t, j, d);
   else
-    launch_densities<float>(P->st, (const Sp<float>*)P->V, N3, m, P->occ[0], P->n, P->h, 1.0, 1, r, t, j, d);
+    launch_densities<float>(P->st, (const Sp<float>*)P->W[0].V, N3, m, P->occ[0], P->n, P->h, 1.0, 1, r, t, j, d);
   hipMemcpyAsync(rho, r, sizeof(double) * N3, hipMemcpyDeviceToHost, P->st);
   hipMemcpyAsync(tau, t, sizeof(double) * N3, hipMemcpyDeviceToHost, P->st);
   hipMemcpyAsync(J9, j, sizeof(double) * 9 * N3, hipMemcpyDeviceToHost, P->st);
@@ -1031,7 +1185,7 @@ int nhf_poisson(void* ctx, int pid, const double* rho_p, double* U, double rel_t
   Problem* P = get_problem(ctx, pid);
   if (!P || !rho_p || !U || max_iter < 0 || !(rel_tol > 0)) return NHF_EINVAL;
   const size_t N3 = P->N3;
-  double* d_rho = (double*)P->V1;
+  double* d_rho = (double*)P->W[0].V1;
   double* d_U = d_rho + N3;
   HIPCHK(hipMemcpyAsync(d_rho, rho_p, sizeof(double) * N3, hipMemcpyHostToDevice, P->st));
   HIPCHK(hipMemcpyAsync(d_U, U, sizeof(double) * N3, hipMemcpyHostToDevice, P->st));
@@ -1040,7 +1194,7 @@ int nhf_poisson(void* ctx, int pid, const double* rho_p, double* U, double rel_t
   HIPCHK(hipMemsetAsync(zeros, 0, sizeof(double) * N3, P->st));
   launch_moments(P->st, P->n, P->a, P->h, zeros, d_rho, 0, nullptr, P->mom, nullptr);
   int it = 0;
-  CHK(poisson_dev(*P, d_rho, d_U, rel_tol, max_iter, &it));
+  CHK(poisson_dev(*P, P->st, d_rho, d_U, rel_tol, max_iter, &it));
   HIPCHK(hipMemcpyAsync(U, d_U, sizeof(double) * N3, hipMemcpyDeviceToHost, P->st));
   HIPCHK(hipStreamSynchronize(P->st));
   if (iters_out) *iters_out = it;
@@ -1060,10 +1214,10 @@ int nhf_gcg_step(void* ctx, int pid, int q, const double* conjdir, int p_cols, c
   std::vector<double> eps;
   GcgInfo info;
   if (P->prec == NHF_PREC_FP64)
-    CHK(gcg_dev<double>(*P, q, (const Sp<double>*)P->psi[q][0], (Sp<double>*)P->psi[q][1],
+    CHK(gcg_dev<double>(*P, P->W[q], q, (const Sp<double>*)P->psi[q][0], (Sp<double>*)P->psi[q][1],
                         (const Sp<double>*)P->conj[q], p_cols, prm, eps, &info));
   else
-    CHK(gcg_dev<float>(*P, q, (const Sp<float>*)P->psi[q][0], (Sp<float>*)P->psi[q][1], (const Sp<float>*)P->conj[q],
+    CHK(gcg_dev<float>(*P, P->W[q], q, (const Sp<float>*)P->psi[q][0], (Sp<float>*)P->psi[q][1], (const Sp<float>*)P->conj[q],
                        p_cols, prm, eps, &info));
   CHK(download_states(*P, P->psi[q][1], psi_inout, m));
   std::copy(eps.begin(), eps.end(), eps_out);
@@ -1113,21 +1267,21 @@ int nhf_bench_hpsi(void* ctx, int pid, int q, int m, int prec, int reps, double*
   // constant so that no denormals or NaNs are streamed
   const size_t cnt = P->N3 * (size_t)m;
   if (prec == NHF_PREC_FP64) {
-    HIPCHK(hipMemsetAsync(P->V, 0, sizeof(Sp<double>) * cnt, P->st));
-    launch_hpsi<double>(P->st, (Sp<double>*)P->V, (Sp<double>*)P->V1, P->N3, m, P->fld64[q], P->n, P->h, 0.0,
+    HIPCHK(hipMemsetAsync(P->W[q].V, 0, sizeof(Sp<double>) * cnt, P->st));
+    launch_hpsi<double>(P->st, (Sp<double>*)P->W[q].V, (Sp<double>*)P->W[q].V1, P->N3, m, P->fld64[q], P->n, P->h, 0.0,
                         nullptr, 0, nullptr);
   } else {
-    HIPCHK(hipMemsetAsync(P->V, 0, sizeof(Sp<float>) * cnt, P->st));
-    launch_hpsi<float>(P->st, (Sp<float>*)P->V, (Sp<float>*)P->V1, P->N3, m, P->fld32[q], P->n, P->h, 0.0, nullptr,
+    HIPCHK(hipMemsetAsync(P->W[q].V, 0, sizeof(Sp<float>) * cnt, P->st));
+    launch_hpsi<float>(P->st, (Sp<float>*)P->W[q].V, (Sp<float>*)P->W[q].V1, P->N3, m, P->fld32[q], P->n, P->h, 0.0, nullptr,
                        0, nullptr);
   }
   HIPCHK(hipEventRecord(e0, P->st));
   for (int r = 0; r < reps; ++r) {
     if (prec == NHF_PREC_FP64)
-      launch_hpsi<double>(P->st, (Sp<double>*)P->V, (Sp<double>*)P->V1, P->N3, m, P->fld64[q], P->n, P->h, 0.0,
+      launch_hpsi<double>(P->st, (Sp<double>*)P->W[q].V, (Sp<double>*)P->W[q].V1, P->N3, m, P->fld64[q], P->n, P->h, 0.0,
                           nullptr, 0, nullptr);
     else
-      launch_hpsi<float>(P->st, (Sp<float>*)P->V, (Sp<float>*)P->V1, P->N3, m, P->fld32[q], P->n, P->h, 0.0, nullptr,
+      launch_hpsi<float>(P->st, (Sp<float>*)P->W[q].V, (Sp<float>*)P->W[q].V1, P->N3, m, P->fld32[q], P->n, P->h, 0.0, nullptr,
                          0, nullptr);
   }
   HIPCHK(hipEventRecord(e1, P->st));
@@ -1137,6 +1291,49 @@ int nhf_bench_hpsi(void* ctx, int pid, int q, int m, int prec, int reps, double*
   *avg_ms = ms / reps;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  return NHF_OK;
+}
+
+int nhf_stencil_stats(void* ctx, int pid, int enable, double* out) {
+  Problem* P = get_problem(ctx, pid);
+  if (!P) return NHF_EINVAL;
+  if (out) {
+    unsigned long long act[2] = {0, 0};
+    for (int q = 0; q < 2; ++q) {
+      HIPCHK(hipMemcpyAsync(&act[q], P->W[q].act, sizeof(unsigned long long), hipMemcpyDeviceToHost, P->st));
+    }
+    HIPCHK(hipStreamSynchronize(P->st));
+    out[0] = P->W[0].st_ms + P->W[1].st_ms;
+    out[1] = (double)(P->W[0].st_launches + P->W[1].st_launches);
+    out[2] = (double)(act[0] + act[1]);
+    out[3] = (double)P->N3;
+  }
+  if (enable >= 0) {
+    for (int q = 0; q < 2; ++q) {
+      P->W[q].timing = enable != 0;
+      P->W[q].st_ms = 0.0;
+      P->W[q].st_launches = 0;
+      HIPCHK(hipMemsetAsync(P->W[q].act, 0, sizeof(unsigned long long), P->st));
+    }
+    HIPCHK(hipStreamSynchronize(P->st));
+  }
+  return NHF_OK;
+}
+
+int nhf_phase_times(void* ctx, int pid, int reset, double* out) {
+  Problem* P = get_problem(ctx, pid);
+  if (!P) return NHF_EINVAL;
+  if (out) {
+    out[0] = P->tcommon;
+    out[1] = P->tenergy;
+    for (int q = 0; q < 2; ++q)
+      for (int k = 0; k < 6; ++k) out[2 + 6 * q + k] = P->W[q].tph[k];
+  }
+  if (reset) {
+    P->tcommon = P->tenergy = 0.0;
+    for (int q = 0; q < 2; ++q)
+      for (double& t : P->W[q].tph) t = 0.0;
+  }
   return NHF_OK;
 }
 

@@ -44,6 +44,8 @@ def lib():
                                   ctypes.c_int]
         L.nhf_bench_hpsi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _D]
+        L.nhf_stencil_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
+        L.nhf_phase_times.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         L.nhf_ho_guess.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]
         L.nhf_eigh_lower.argtypes = [ctypes.c_int, _D, _D, _D]
         L.nhf_cholqr_coeffs.argtypes = [ctypes.c_int, _D, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _D]
@@ -197,6 +199,22 @@ class Problem:
         _chk(lib().nhf_bench_hpsi(self.ctx.h, self.pid, q, m, self.cfg.precision if prec is None else prec, reps,
                                   ctypes.byref(ms)))
         return ms.value
+
+    def stencil_stats(self, enable=-1):
+        out = np.zeros(4)
+        _chk(lib().nhf_stencil_stats(self.ctx.h, self.pid, enable, _dp(out)))
+        return {"ms": out[0], "launches": int(out[1]), "active_cols": int(out[2]), "N3": int(out[3])}
+
+    PHASES = ["fields", "orth+RR1", "CG", "basis+CholQR", "RR2", "norm/TR/conj"]
+
+    def phase_times(self, reset=False):
+        out = np.zeros(14)
+        _chk(lib().nhf_phase_times(self.ctx.h, self.pid, int(reset), _dp(out)))
+        d = {"common": out[0], "energy": out[1]}
+        for q, sp in enumerate("np"):
+            for k, name in enumerate(self.PHASES):
+                d[f"{sp}:{name}"] = out[2 + 6 * q + k]
+        return d
 
     def scf_run(self, psi_n, psi_p, eps_n, eps_p, trace_cap=4000):
         psi_n = _cols(psi_n).copy(order="F")
