@@ -1,0 +1,48 @@
+"""Ca-40 40^3 HF+BCS (the benchmark workload) on the GPU against the FP64
+CPU oracle's converged result committed in tests/golden/ca40_oracle.json
+(tests/golden/make_ca40_oracle.py)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import bench
+from nuclearcgm_amd import config as C
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ca40_oracle.json")))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from nuclearcgm_amd import hip
+
+    c = hip.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("prec,tol_kev", [(C.PREC_FP64, 0.01), (C.PREC_MIXED, 1.0)])
+def test_ca40_matches_oracle(ctx, prec, tol_kev):
+    # FP64 path: converged E_tot within 10 eV of the oracle (both stop at
+    # |dE| < 1e-6 MeV; the device reductions use atomics, so runs differ in
+    # the last bits); accelerated FP32 path: within the 1 keV target.
+    cfg = C.workload("ca40", precision=prec)
+    cfg.max_iter = 600
+    g = bench.make_guess(cfg)
+    pn, pp, en, ep, res, trace = ctx.problem(cfg).scf_run(*g)
+    s = res.summary()
+    dE = (s["E_int"] - GOLD["E_int"]) * 1e3
+    print(f"prec {prec}: E {s['E_int']:.7f} dE {dE:.4f} keV, its {s['iterations']}, {s['seconds']:.3f} s")
+    assert s["converged"]
+    assert abs(dE) < tol_kev
+    # the iteration-0 energy depends only on the guess
+    assert abs(trace[0, 0] - GOLD["trace_E_int"][0]) < 1e-8
+    assert np.abs(en - np.array(GOLD["eps_n"])).max() < (1e-4 if prec == C.PREC_FP64 else 1e-3)
+    assert np.abs(ep - np.array(GOLD["eps_p"])).max() < (1e-4 if prec == C.PREC_FP64 else 1e-3)
+    assert abs(s["rms_n"] - GOLD["rms_n"]) < 1e-5 and abs(s["rms_p"] - GOLD["rms_p"]) < 1e-5
+    assert abs(s["beta2"] - GOLD["beta2"]) < 1e-5
+    assert abs(s["lambda_n"] - GOLD["lambda_n"]) < 1e-3 and abs(s["lambda_p"] - GOLD["lambda_p"]) < 1e-3
