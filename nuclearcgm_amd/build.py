@@ -18,7 +18,7 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
     # no packed-FP32 instructions in any HIP code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-fcx-limited-range",
+    "-fcx-limited-range", "-Xarch_host", "-march=x86-64-v3",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value",
 ]
 

@@ -16,70 +16,15 @@
 
 namespace nhf {
 
-bool eigh_lower(int k, const cd* Ain, double* w, cd* V) {
-  if (k <= 0) return false;
-  std::vector<cd> A((size_t)k * k);
-  auto a = [&](int r, int c) -> cd& { return A[(size_t)c * k + r]; };
-  for (int c = 0; c < k; ++c)
-    for (int r = 0; r < k; ++r) {
-      if (r > c)
-        a(r, c) = Ain[(size_t)c * k + r];
-      else if (r < c)
-        a(r, c) = std::conj(Ain[(size_t)r * k + c]);
-      else
-        a(r, c) = cd(Ain[(size_t)c * k + r].real(), 0.0);
-    }
-  std::vector<double> d(k, 0.0), e(k, 0.0);
-  std::vector<cd> tau(k, 0.0);
-  std::vector<std::vector<cd>> vs(k);
-  std::vector<cd> x(k);
-  for (int i = 0; i < k - 1; ++i) {
-    const int m = k - i - 1;  // length of the reflected vector A(i+1:k, i)
-    cd alpha = a(i + 1, i);
-    double xn2 = 0.0;
-    for (int r = i + 2; r < k; ++r) xn2 += std::norm(a(r, i));
-    cd ti = 0.0;
-    std::vector<cd> v(m, 0.0);
-    v[0] = 1.0;
-    if (!(xn2 == 0.0 && alpha.imag() == 0.0)) {
-      double beta = -std::copysign(std::sqrt(std::norm(alpha) + xn2), alpha.real());
-      ti = cd((beta - alpha.real()) / beta, -alpha.imag() / beta);
-      cd sc = 1.0 / (alpha - beta);
-      for (int r = i + 2; r < k; ++r) v[r - i - 1] = a(r, i) * sc;
-      alpha = beta;
-    }
-    e[i] = alpha.real();
-    if (ti != cd(0.0, 0.0)) {
-      // x = tau A22 v ; x += (-1/2 tau x^H v) v ; A22 -= v x^H + x v^H
-      for (int r = 0; r < m; ++r) {
-        cd s = 0.0;
-        for (int c = 0; c < m; ++c) s += a(i + 1 + r, i + 1 + c) * v[c];
-        x[r] = ti * s;
-      }
-      cd xv = 0.0;
-      for (int r = 0; r < m; ++r) xv += std::conj(x[r]) * v[r];
-      cd al2 = -0.5 * ti * xv;
-      for (int r = 0; r < m; ++r) x[r] += al2 * v[r];
-      for (int c = 0; c < m; ++c) {
-        cd xc = std::conj(x[c]), vc = std::conj(v[c]);
-        for (int r = 0; r < m; ++r) a(i + 1 + r, i + 1 + c) -= v[r] * xc + x[r] * vc;
-      }
-    }
-    a(i + 1, i + 1) = cd(a(i + 1, i + 1).real(), 0.0);
-    d[i] = a(i, i).real();
-    tau[i] = ti;
-    vs[i] = std::move(v);
-  }
-  d[k - 1] = a(k - 1, k - 1).real();
+// implicit QL with Wilkinson shifts on the symmetric tridiagonal (d, e),
+// e[i] coupling i and i+1, eigenvalues only (returned in d, unsorted)
+static bool tql_values(int k, double* d, double* e) {
   e[k - 1] = 0.0;
-  // implicit QL on (d, e), eigenvectors accumulated in Z (column-major)
-  std::vector<double> Z((size_t)k * k, 0.0);
-  for (int i = 0; i < k; ++i) Z[(size_t)i * k + i] = 1.0;
   for (int l = 0; l < k; ++l) {
     int iter = 0, mm;
     do {
       for (mm = l; mm < k - 1; ++mm) {
-        double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
+        const double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
         if (std::fabs(e[mm]) <= DBL_EPSILON * dd) break;
       }
       if (mm != l) {
@@ -90,7 +35,7 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V) {
         double s = 1.0, c = 1.0, p = 0.0;
         int i;
         for (i = mm - 1; i >= l; --i) {
-          double f = s * e[i], b = c * e[i];
+          const double f = s * e[i], b = c * e[i];
           e[i + 1] = (r = std::hypot(f, g));
           if (r == 0.0) {
             d[i + 1] -= p;
@@ -103,13 +48,6 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V) {
           r = (d[i] - g) * s + 2.0 * c * b;
           d[i + 1] = g + (p = s * r);
           g = c * r - b;
-          double* zi = &Z[(size_t)i * k];
-          double* zi1 = &Z[(size_t)(i + 1) * k];
-          for (int q = 0; q < k; ++q) {
-            f = zi1[q];
-            zi1[q] = s * zi[q] + c * f;
-            zi[q] = c * zi[q] - s * f;
-          }
         }
         if (r == 0.0 && i >= l) continue;
         d[l] -= p;
@@ -118,24 +56,219 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V) {
       }
     } while (mm != l);
   }
-  // back-transformation V = H(0) H(1) ... H(k-2) Z, ascending order
-  std::vector<int> ord(k);
-  std::iota(ord.begin(), ord.end(), 0);
-  std::stable_sort(ord.begin(), ord.end(), [&](int x1, int x2) { return d[x1] < d[x2]; });
-  for (int c = 0; c < k; ++c) {
-    w[c] = d[ord[c]];
-    cd* col = V + (size_t)c * k;
-    const double* z = &Z[(size_t)ord[c] * k];
-    for (int r = 0; r < k; ++r) col[r] = z[r];
+  return true;
+}
+
+// Inverse iteration for eigenvectors of the symmetric tridiagonal (d, e) at
+// the given ascending eigenvalues lam[0..nv): Gaussian elimination with
+// partial pivoting of T - lam I, three solves from a fixed pseudo-random
+// start, Gram-Schmidt against earlier vectors of the same cluster
+// (eigenvalues closer than 1e-3 |T|), in the manner of LAPACK's stein.
+static bool tri_inverse_iteration(int k, const double* d, const double* e, const double* lam, int nv, double* Z) {
+  double tn = 0.0;
+  for (int i = 0; i < k; ++i)
+    tn = std::max(tn, std::fabs(d[i]) + (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(e[i]) : 0.0));
+  if (tn == 0.0) tn = 1.0;
+  const double ortol = 1e-3 * tn, pertol = 10.0 * DBL_EPSILON * tn, tiny = DBL_EPSILON * tn;
+  std::vector<double> u0(k), u1(k), u2(k), lm(k), x(k);
+  std::vector<char> piv(k);
+  int cl0 = 0;  // first vector of the current cluster
+  double prev = -1e300;
+  for (int j = 0; j < nv; ++j) {
+    double lj = lam[j];
+    if (j > 0 && lam[j] - lam[j - 1] > ortol) cl0 = j;
+    if (j > cl0 && lj - prev < pertol) lj = prev + pertol;
+    prev = lj;
+    // factor T - lj I
+    double a0 = d[0] - lj, a1 = k > 1 ? e[0] : 0.0, a2 = 0.0;
+    for (int i = 0; i < k - 1; ++i) {
+      const double bs = e[i], b0 = d[i + 1] - lj, b1 = (i + 2 < k) ? e[i + 1] : 0.0;
+      if (std::fabs(bs) > std::fabs(a0)) {
+        piv[i] = 1;
+        const double m = a0 / bs;
+        lm[i] = m;
+        u0[i] = bs;
+        u1[i] = b0;
+        u2[i] = b1;
+        a0 = a1 - m * b0;
+        a1 = a2 - m * b1;
+        a2 = 0.0;
+      } else {
+        piv[i] = 0;
+        const double m = a0 != 0.0 ? bs / a0 : 0.0;
+        lm[i] = m;
+        u0[i] = a0;
+        u1[i] = a1;
+        u2[i] = a2;
+        a0 = b0 - m * a1;
+        a1 = b1 - m * a2;
+        a2 = 0.0;
+      }
+    }
+    u0[k - 1] = a0;
+    u1[k - 1] = 0.0;
+    u2[k - 1] = 0.0;
+    for (int i = 0; i < k; ++i)
+      if (std::fabs(u0[i]) < tiny) u0[i] = std::copysign(tiny, u0[i] == 0.0 ? 1.0 : u0[i]);
+    // fixed pseudo-random start
+    uint64_t st = 0x9E3779B97F4A7C15ull * (uint64_t)(j + 1);
+    for (int i = 0; i < k; ++i) {
+      st ^= st >> 12;
+      st ^= st << 25;
+      st ^= st >> 27;
+      x[i] = (double)((st * 2685821657736338717ull) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    for (int it = 0; it < 3; ++it) {
+      for (int i = 0; i < k - 1; ++i) {
+        if (piv[i]) std::swap(x[i], x[i + 1]);
+        x[i + 1] -= lm[i] * x[i];
+      }
+      for (int i = k - 1; i >= 0; --i) {
+        double s = x[i];
+        if (i + 1 < k) s -= u1[i] * x[i + 1];
+        if (i + 2 < k) s -= u2[i] * x[i + 2];
+        x[i] = s / u0[i];
+      }
+      for (int pass = 0; pass < 2; ++pass)
+        for (int q = cl0; q < j; ++q) {
+          const double* zq = Z + (size_t)q * k;
+          double dt = 0.0;
+          for (int i = 0; i < k; ++i) dt += zq[i] * x[i];
+          for (int i = 0; i < k; ++i) x[i] -= dt * zq[i];
+        }
+      double nr = 0.0;
+      for (int i = 0; i < k; ++i) nr += x[i] * x[i];
+      nr = std::sqrt(nr);
+      if (!(nr > 0.0) || !std::isfinite(nr)) return false;
+      for (int i = 0; i < k; ++i) x[i] /= nr;
+    }
+    std::copy(x.begin(), x.end(), Z + (size_t)j * k);
+  }
+  return true;
+}
+
+bool eigh_lower(int k, const cd* Ain, double* w, cd* V, int nvec) {
+  if (k <= 0) return false;
+  if (nvec < 0 || nvec > k) nvec = k;
+  // full Hermitian matrix from the lower triangle, split real / imaginary
+  // parts (column-major) so that the O(k^3) loops vectorise
+  std::vector<double> Ar((size_t)k * k), Ai((size_t)k * k);
+  for (int c = 0; c < k; ++c)
+    for (int r = 0; r < k; ++r) {
+      cd v;
+      if (r > c)
+        v = Ain[(size_t)c * k + r];
+      else if (r < c)
+        v = std::conj(Ain[(size_t)r * k + c]);
+      else
+        v = cd(Ain[(size_t)c * k + r].real(), 0.0);
+      Ar[(size_t)c * k + r] = v.real();
+      Ai[(size_t)c * k + r] = v.imag();
+    }
+  std::vector<double> d(k, 0.0), e(k, 0.0);
+  std::vector<cd> tau(k, 0.0);
+  std::vector<double> vsr((size_t)k * k, 0.0), vsi((size_t)k * k, 0.0);  // reflector i in column i
+  std::vector<double> xr(k), xi(k);
+  for (int i = 0; i < k - 1; ++i) {
+    const int m = k - i - 1, o = i + 1;
+    double* vr = &vsr[(size_t)i * k];
+    double* vi = &vsi[(size_t)i * k];
+    cd alpha(Ar[(size_t)i * k + o], Ai[(size_t)i * k + o]);
+    double xn2 = 0.0;
+    for (int r = o + 1; r < k; ++r) xn2 += Ar[(size_t)i * k + r] * Ar[(size_t)i * k + r] + Ai[(size_t)i * k + r] * Ai[(size_t)i * k + r];
+    cd ti = 0.0;
+    vr[0] = 1.0;
+    vi[0] = 0.0;
+    if (!(xn2 == 0.0 && alpha.imag() == 0.0)) {
+      const double beta = -std::copysign(std::sqrt(std::norm(alpha) + xn2), alpha.real());
+      ti = cd((beta - alpha.real()) / beta, -alpha.imag() / beta);
+      const cd sc = 1.0 / (alpha - beta);
+      for (int r = o + 1; r < k; ++r) {
+        const cd a(Ar[(size_t)i * k + r], Ai[(size_t)i * k + r]);
+        const cd v = a * sc;
+        vr[r - o] = v.real();
+        vi[r - o] = v.imag();
+      }
+      alpha = beta;
+    }
+    e[i] = alpha.real();
+    if (ti != cd(0.0, 0.0)) {
+      // y = A22 v (column sweep), x = tau y
+      std::fill(xr.begin(), xr.begin() + m, 0.0);
+      std::fill(xi.begin(), xi.begin() + m, 0.0);
+      for (int c = 0; c < m; ++c) {
+        const double* ar = &Ar[(size_t)(o + c) * k + o];
+        const double* ai = &Ai[(size_t)(o + c) * k + o];
+        const double br = vr[c], bi = vi[c];
+        for (int r = 0; r < m; ++r) {
+          xr[r] += ar[r] * br - ai[r] * bi;
+          xi[r] += ar[r] * bi + ai[r] * br;
+        }
+      }
+      double xvr = 0.0, xvi = 0.0;  // (tau y)^H v
+      for (int r = 0; r < m; ++r) {
+        const double tr = ti.real() * xr[r] - ti.imag() * xi[r], tim = ti.real() * xi[r] + ti.imag() * xr[r];
+        xr[r] = tr;
+        xi[r] = tim;
+        xvr += tr * vr[r] + tim * vi[r];
+        xvi += tr * vi[r] - tim * vr[r];
+      }
+      const cd al2 = -0.5 * ti * cd(xvr, xvi);
+      for (int r = 0; r < m; ++r) {
+        xr[r] += al2.real() * vr[r] - al2.imag() * vi[r];
+        xi[r] += al2.real() * vi[r] + al2.imag() * vr[r];
+      }
+      // A22 -= v x^H + x v^H
+      for (int c = 0; c < m; ++c) {
+        double* ar = &Ar[(size_t)(o + c) * k + o];
+        double* ai = &Ai[(size_t)(o + c) * k + o];
+        const double xcr = xr[c], xci = -xi[c], vcr = vr[c], vci = -vi[c];  // conjugates
+        for (int r = 0; r < m; ++r) {
+          ar[r] -= vr[r] * xcr - vi[r] * xci + xr[r] * vcr - xi[r] * vci;
+          ai[r] -= vr[r] * xci + vi[r] * xcr + xr[r] * vci + xi[r] * vcr;
+        }
+      }
+    }
+    Ai[(size_t)o * k + o] = 0.0;
+    d[i] = Ar[(size_t)i * k + i];
+    tau[i] = ti;
+  }
+  d[k - 1] = Ar[(size_t)(k - 1) * k + k - 1];
+  e[k - 1] = 0.0;
+  // eigenvalues: implicit QL without vectors on a copy of (d, e)
+  std::vector<double> dv(d), ev(e);
+  if (!tql_values(k, dv.data(), ev.data())) return false;
+  std::sort(dv.begin(), dv.end());
+  for (int c = 0; c < k; ++c) w[c] = dv[c];
+  if (nvec == 0) return std::isfinite(w[0]);
+  // eigenvectors of the tridiagonal matrix for the nvec lowest eigenvalues
+  std::vector<double> Z((size_t)k * nvec);
+  if (!tri_inverse_iteration(k, d.data(), e.data(), dv.data(), nvec, Z.data())) return false;
+  std::vector<double> cr(k), ci(k);
+  for (int c = 0; c < nvec; ++c) {
+    const double* z = &Z[(size_t)c * k];
+    for (int r = 0; r < k; ++r) {
+      cr[r] = z[r];
+      ci[r] = 0.0;
+    }
     for (int i = k - 2; i >= 0; --i) {
       if (tau[i] == cd(0.0, 0.0)) continue;
-      const std::vector<cd>& v = vs[i];
-      const int m = k - i - 1;
-      cd s = 0.0;
-      for (int r = 0; r < m; ++r) s += std::conj(v[r]) * col[i + 1 + r];
-      s *= tau[i];
-      for (int r = 0; r < m; ++r) col[i + 1 + r] -= v[r] * s;
+      const double* vr = &vsr[(size_t)i * k];
+      const double* vi = &vsi[(size_t)i * k];
+      const int m = k - i - 1, o = i + 1;
+      double sr = 0.0, si = 0.0;  // v^H col
+      for (int r = 0; r < m; ++r) {
+        sr += vr[r] * cr[o + r] + vi[r] * ci[o + r];
+        si += vr[r] * ci[o + r] - vi[r] * cr[o + r];
+      }
+      const cd t = tau[i] * cd(sr, si);
+      for (int r = 0; r < m; ++r) {
+        cr[o + r] -= vr[r] * t.real() - vi[r] * t.imag();
+        ci[o + r] -= vr[r] * t.imag() + vi[r] * t.real();
+      }
     }
+    cd* col = V + (size_t)c * k;
+    for (int r = 0; r < k; ++r) col[r] = cd(cr[r], ci[r]);
   }
   for (int c = 0; c < k; ++c)
     if (!std::isfinite(w[c])) return false;
@@ -149,45 +282,48 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V) {
 // matrix).  Cinv (k x ke, column-major) = rows `keep` of R^-1.
 int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, std::vector<cd>& Cinv,
                   double* min_rel) {
-  double mr = 1.0;
   auto g = [&](int r, int c) -> cd { return r >= c ? G[(size_t)c * k + r] : std::conj(G[(size_t)r * k + c]); };
   keep.clear();
-  std::vector<cd> R;  // column-major ke x ke upper triangular, grown column by column
-  std::vector<std::vector<cd>> cols;
+  double mr = 1.0;
+  // R (ke x ke upper, column-major with leading dimension k) grown column by
+  // column: R^H R = G[keep, keep]
+  std::vector<cd> R((size_t)k * k, 0.0), rj(k);
+  int ke = 0;
   for (int j = 0; j < k; ++j) {
-    const int ke = (int)keep.size();
-    std::vector<cd> rj(ke + 1, 0.0);
     double s = 0.0;
     for (int i = 0; i < ke; ++i) {
+      const cd* Ri = &R[(size_t)i * k];  // column i of R, rows 0..i
       cd v = g(keep[i], j);
-      for (int l = 0; l < i; ++l) v -= std::conj(cols[i][l]) * rj[l];
-      v /= cols[i][i];
+      for (int l = 0; l < i; ++l) v -= std::conj(Ri[l]) * rj[l];
+      v /= Ri[i].real();
       rj[i] = v;
       s += std::norm(v);
     }
-    double gjj = g(j, j).real();
-    double dd = gjj - s;
+    const double gjj = g(j, j).real();
+    const double dd = gjj - s;
     if (!(dd > drop_rel * gjj) || !(gjj > 0.0)) continue;
     mr = std::min(mr, dd / gjj);
-    rj[ke] = std::sqrt(dd);
+    cd* Rc = &R[(size_t)ke * k];
+    for (int i = 0; i < ke; ++i) Rc[i] = rj[i];
+    Rc[ke] = std::sqrt(dd);
     keep.push_back(j);
-    cols.push_back(rj);
+    ++ke;
   }
-  const int ke = (int)keep.size();
   if (min_rel) *min_rel = mr;
-  // R^-1 by back substitution, column by column
-  std::vector<cd> Ri((size_t)ke * ke, 0.0);
+  // R^-1 by back substitution, column by column (X upper, ld k)
+  std::vector<cd> X((size_t)k * k, 0.0);
   for (int c = 0; c < ke; ++c) {
-    Ri[(size_t)c * ke + c] = 1.0 / cols[c][c];
+    cd* Xc = &X[(size_t)c * k];
+    Xc[c] = 1.0 / R[(size_t)c * k + c].real();
     for (int r = c - 1; r >= 0; --r) {
-      cd s = 0.0;
-      for (int l = r + 1; l <= c; ++l) s += cols[l][r] * Ri[(size_t)c * ke + l];
-      Ri[(size_t)c * ke + r] = -s / cols[r][r];
+      cd sum = 0.0;
+      for (int l = r + 1; l <= c; ++l) sum += R[(size_t)l * k + r] * Xc[l];
+      Xc[r] = -sum / R[(size_t)r * k + r].real();
     }
   }
   Cinv.assign((size_t)k * ke, 0.0);
   for (int c = 0; c < ke; ++c)
-    for (int r = 0; r <= c; ++r) Cinv[(size_t)c * k + keep[r]] = Ri[(size_t)c * ke + r];
+    for (int r = 0; r <= c; ++r) Cinv[(size_t)c * k + keep[r]] = X[(size_t)c * k + r];
   return ke;
 }
 

@@ -155,7 +155,8 @@ void launch_r2(hipStream_t st, int n, double a, double h, const double* rho_n, c
 // Hermitian eigensolver reading the lower triangle of A (k x k, column
 // major), ascending eigenvalues.  Householder tridiagonalisation + implicit
 // QL.  Returns false on failure.
-bool eigh_lower(int k, const cd* A, double* w, cd* V);
+// Only the first nvec (lowest) eigenvectors are formed (nvec < 0: all).
+bool eigh_lower(int k, const cd* A, double* w, cd* V, int nvec = -1);
 // Cholesky-QR coefficients with column dropping: given a Gram matrix G
 // (k x k Hermitian, lower triangle used) return the kept column list and
 // Cinv (k x ke) with V_eff = V * Cinv orthonormal.  drop_rel: a column is

@@ -330,7 +330,7 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
   CHK(gram_host<R>(P, Wk, V1, M, AV, M, Pm));
   std::vector<double> w(M);
   std::vector<cd> Cx((size_t)M * M);
-  if (!eigh_lower(M, Pm.data(), w.data(), Cx.data())) return NHF_ESOLVER;
+  if (!eigh_lower(M, Pm.data(), w.data(), Cx.data(), M)) return NHF_ESOLVER;
   CHK(rotate_dev<R>(P, Wk, V1, M, Cx, M, V, nullptr, nullptr, false));  // X -> V[0:M]
   std::vector<double> Lam = w;
   double shift = 0.0;
@@ -466,8 +466,8 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
     }
     std::vector<double> wk(k2);
     std::vector<cd> Ck((size_t)k2 * k2);
-    if (!eigh_lower(k2, Pm.data(), wk.data(), Ck.data())) return NHF_ESOLVER;
     const int rr = std::min(M, k2);
+    if (!eigh_lower(k2, Pm.data(), wk.data(), Ck.data(), rr)) return NHF_ESOLVER;
     std::vector<cd> Ctot((size_t)k1 * rr, 0.0);
     if (pass2) {
       for (int c = 0; c < rr; ++c)

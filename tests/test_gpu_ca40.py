@@ -39,10 +39,14 @@ def test_ca40_matches_oracle(ctx, prec, tol_kev):
     print(f"prec {prec}: E {s['E_int']:.7f} dE {dE:.4f} keV, its {s['iterations']}, {s['seconds']:.3f} s")
     assert s["converged"]
     assert abs(dE) < tol_kev
-    # the iteration-0 energy depends only on the guess
-    assert abs(trace[0, 0] - GOLD["trace_E_int"][0]) < 1e-8
+    # the iteration-0 energy depends only on the guess (stored in FP32 on the
+    # accelerated path: densities to ~1e-7 relative)
+    assert abs(trace[0, 0] - GOLD["trace_E_int"][0]) < (1e-8 if prec == C.PREC_FP64 else 1e-5)
     assert np.abs(en - np.array(GOLD["eps_n"])).max() < (1e-4 if prec == C.PREC_FP64 else 1e-3)
     assert np.abs(ep - np.array(GOLD["eps_p"])).max() < (1e-4 if prec == C.PREC_FP64 else 1e-3)
     assert abs(s["rms_n"] - GOLD["rms_n"]) < 1e-5 and abs(s["rms_p"] - GOLD["rms_p"]) < 1e-5
     assert abs(s["beta2"] - GOLD["beta2"]) < 1e-5
-    assert abs(s["lambda_n"] - GOLD["lambda_n"]) < 1e-3 and abs(s["lambda_p"] - GOLD["lambda_p"]) < 1e-3
+    # BCS chemical potentials (MeV): the runs stop on |dE| < 1e-6 MeV, which
+    # leaves lambda converged only to ~1e-4 (FP64) and ~1e-3 (FP32)
+    ltol = 1e-3 if prec == C.PREC_FP64 else 3e-3
+    assert abs(s["lambda_n"] - GOLD["lambda_n"]) < ltol and abs(s["lambda_p"] - GOLD["lambda_p"]) < ltol
