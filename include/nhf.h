@@ -119,7 +119,31 @@ typedef struct nhf_config {
   int precision;         /* NHF_PREC_* */
   double stop_energy;    /* if finite: stop once |E_int - stop_energy| <= stop_window */
   double stop_window;    /* MeV (1e-3 = 1 keV); ignored when stop_energy is NaN */
+  /* Accelerated-path options.  All zero = the reference iteration. */
+  int mixing;            /* NHF_MIX_LINEAR (reference, density_mix) | NHF_MIX_BROYDEN */
+  int mix_hist;          /* Broyden history length, 1..NHF_MAX_HIST (0 -> 8) */
+  double mix_alpha;      /* Broyden linear part (0 -> 0.5) */
+  int wstep;             /* NHF_W_JACOBI_CG (reference) | NHF_W_LEARNED */
+  int wstep_iters;       /* learned W-step: flexible-PCG iterations (0 -> 1) */
 } nhf_config;
+
+/* Density mixing.  The reference mixes linearly (iteration_data.cpp:928-995);
+ * its input parser reads a `useDIIS` flag (input_parser.cpp:199-200) that it
+ * never acts on.  NHF_MIX_BROYDEN is the modified Broyden method of
+ * D. D. Johnson, Phys. Rev. B 38, 12807 (1988) on the vector of all mixed
+ * densities (rho, tau, J, div J of both species); it has the same fixed
+ * point as linear mixing. */
+#define NHF_MIX_LINEAR 0
+#define NHF_MIX_BROYDEN 1
+#define NHF_MAX_HIST 16
+
+/* The GCG W-block.  NHF_W_JACOBI_CG is the reference's Jacobi-PCG solve of
+ * (H + sigma) w = (lambda + sigma) x (solver.cpp:727-745); NHF_W_LEARNED
+ * replaces the Jacobi preconditioner by the context's learned 3D
+ * convolutional preconditioner (nhf_lcp_*) and runs wstep_iters iterations,
+ * falling back to NHF_W_JACOBI_CG for a species whose residual stalls. */
+#define NHF_W_JACOBI_CG 0
+#define NHF_W_LEARNED 1
 
 /* Terms of the total energy (reference iteration_data.cpp:153-271 and
  * iteration_data_integrals.cpp:95-262). */

@@ -23,6 +23,8 @@ MNUC = 938.91869   # MeV (constants.hpp:14)
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+MIX_LINEAR, MIX_BROYDEN = 0, 1
+W_JACOBI_CG, W_LEARNED = 0, 1
 PAIR_NONE, PAIR_BCS, PAIR_HFB = 0, 1, 2
 CON_XCM, CON_YCM, CON_ZCM, CON_X2MY2, CON_XY2, CON_OCT, CON_Q20 = range(7)
 MAX_CON = 8
@@ -56,6 +58,8 @@ class NhfConfig(ctypes.Structure):
         ("start_hfb_iter", ctypes.c_int), ("pairing_threshold", ctypes.c_double), ("n_con", ctypes.c_int),
         ("con", NhfConstraint * MAX_CON), ("constraints_tol", ctypes.c_double), ("precision", ctypes.c_int),
         ("stop_energy", ctypes.c_double), ("stop_window", ctypes.c_double),
+        ("mixing", ctypes.c_int), ("mix_hist", ctypes.c_int), ("mix_alpha", ctypes.c_double),
+        ("wstep", ctypes.c_int), ("wstep_iters", ctypes.c_int),
     ]
 
     @property

@@ -27,6 +27,11 @@ constexpr int NFLD = 8;
 // Scalars accumulated per CG iteration and column:
 // [0..1] p^H A p (re, im), [2] |r|^2, [3] r^H D^-1 r.
 constexpr int CG_NS = 4;
+// history slots of the Broyden density mixer (NHF_MAX_HIST)
+constexpr int BROYDEN_MAX = 16;
+struct BroydenCoef {
+  double g[BROYDEN_MAX];
+};
 
 // ------------------------------------------------------------------ kernels
 // All launches go to `st`.  launch_hpsi: out = (H + shift) in; when dot is
@@ -164,5 +169,13 @@ bool eigh_lower(int k, const cd* A, double* w, cd* V, int nvec = -1);
 // min_rel (optional): smallest kept d_jj / G_jj, a conditioning measure.
 int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, std::vector<cd>& Cinv,
                   double* min_rel = nullptr);
+
+// modified Broyden density mixing (two passes; see nhf_kernels.hip)
+void launch_broyden_diff(hipStream_t st, size_t D, const double* xout, const double* x, double* Fprev,
+                         double* xprev, double* hF, double* hx, int L, int s, unsigned valid, int have_prev,
+                         double* dots);
+void launch_broyden_update(hipStream_t st, size_t D, double* x, const double* F, const double* hF,
+                           const double* hx, int L, unsigned valid, const BroydenCoef& g, double alpha,
+                           size_t nclamp);
 
 }  // namespace nhf

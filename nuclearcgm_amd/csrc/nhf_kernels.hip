@@ -885,12 +885,20 @@ __global__ __launch_bounds__(BS) void k_densities(const Sp<R>* __restrict__ psi,
     // div J = -2 [Im(g_y^+ sz g_x) + Im(g_z^+ sx g_y) + Im(g_x^+ sy g_z)]
     dj += -2.0 * w * (sig_prod(g[1], 2, g[0]).y + sig_prod(g[2], 0, g[1]).y + sig_prod(g[0], 1, g[2]).y);
   }
-  const double a_ = first ? 1.0 : mu, b_ = first ? 0.0 : 1.0 - mu;
-  rho[p] = b_ * rho[p] + a_ * r;
-  tau[p] = b_ * tau[p] + a_ * t;
-  divJ[p] = b_ * divJ[p] + a_ * dj;
+  if (first) {  // no read of the old values (the buffer may be uninitialised)
+    rho[p] = r;
+    tau[p] = t;
+    divJ[p] = dj;
 #pragma unroll
-  for (int q = 0; q < 9; ++q) J9[(size_t)q * N3 + p] = b_ * J9[(size_t)q * N3 + p] + a_ * J[q];
+    for (int q = 0; q < 9; ++q) J9[(size_t)q * N3 + p] = J[q];
+    return;
+  }
+  const double b_ = 1.0 - mu;
+  rho[p] = b_ * rho[p] + mu * r;
+  tau[p] = b_ * tau[p] + mu * t;
+  divJ[p] = b_ * divJ[p] + mu * dj;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) J9[(size_t)q * N3 + p] = b_ * J9[(size_t)q * N3 + p] + mu * J[q];
 }
 template <typename R>
 void launch_densities(hipStream_t st, const Sp<R>* psi, size_t ld, int m, const double* occ, int n, double h,
@@ -1443,6 +1451,78 @@ __global__ __launch_bounds__(BS) void k_r2(int n, double a, double h, const doub
 void launch_r2(hipStream_t st, int n, double a, double h, const double* rho_n, const double* rho_p, double* out) {
   size_t N3 = (size_t)n * n * n;
   hipLaunchKernelGGL(k_r2, dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, n, a, h, rho_n, rho_p, out);
+}
+
+// ------------------------------------------------------ Broyden mixing
+// Modified Broyden step (Johnson 1988) over the D-element vector of mixed
+// densities.  Pass 1: F = xout - x; with a previous step the differences
+// dF = F - Fprev and dx = x - xprev go into history slot s, together with
+// the dot products <dF_i, dF>, <dF_i, F> (valid slots i != s), <dF, dF>,
+// <dF, F> and <F, F>; then Fprev = F and xprev = x.  The dots land in
+// dots[0..L) , dots[L..2L), dots[2L], dots[2L+1], dots[2L+2].
+constexpr int BRO_NV = 2 * BROYDEN_MAX + 3;
+__global__ __launch_bounds__(BS) void k_broyden_diff(size_t D, const double* __restrict__ xout,
+                                                     const double* __restrict__ x, double* __restrict__ Fprev,
+                                                     double* __restrict__ xprev, double* __restrict__ hF,
+                                                     double* __restrict__ hx, int L, int s, unsigned valid,
+                                                     int have_prev, double* __restrict__ dots) {
+  double acc[BRO_NV];
+#pragma unroll
+  for (int k = 0; k < BRO_NV; ++k) acc[k] = 0.0;
+  for (size_t e = (size_t)blockIdx.x * BS + threadIdx.x; e < D; e += (size_t)gridDim.x * BS) {
+    const double xe = x[e];
+    const double F = xout[e] - xe;
+    if (have_prev) {
+      const double dF = F - Fprev[e];
+      hF[(size_t)s * D + e] = dF;
+      hx[(size_t)s * D + e] = xe - xprev[e];
+#pragma unroll
+      for (int i = 0; i < BROYDEN_MAX; ++i) {
+        if (i >= L || !((valid >> i) & 1u)) continue;
+        const double v = hF[(size_t)i * D + e];
+        acc[i] += v * dF;
+        acc[BROYDEN_MAX + i] += v * F;
+      }
+      acc[2 * BROYDEN_MAX] += dF * dF;
+      acc[2 * BROYDEN_MAX + 1] += dF * F;
+    }
+    acc[2 * BROYDEN_MAX + 2] += F * F;
+    Fprev[e] = F;
+    xprev[e] = xe;
+  }
+  block_atomic_add<BRO_NV>(acc, dots);
+}
+// Pass 2: x += alpha F - sum_l g_l (alpha dF_l + dx_l) over the valid slots;
+// the first `nclamp` elements (densities rho, tau) are kept >= 0.
+__global__ __launch_bounds__(BS) void k_broyden_update(size_t D, double* __restrict__ x,
+                                                       const double* __restrict__ F, const double* __restrict__ hF,
+                                                       const double* __restrict__ hx, int L, unsigned valid,
+                                                       BroydenCoef g, double alpha, size_t nclamp) {
+  for (size_t e = (size_t)blockIdx.x * BS + threadIdx.x; e < D; e += (size_t)gridDim.x * BS) {
+    double v = x[e] + alpha * F[e];
+#pragma unroll
+    for (int i = 0; i < BROYDEN_MAX; ++i) {
+      if (i >= L || !((valid >> i) & 1u)) continue;
+      v -= g.g[i] * (alpha * hF[(size_t)i * D + e] + hx[(size_t)i * D + e]);
+    }
+    if (e < nclamp) v = fmax(v, 0.0);
+    x[e] = v;
+  }
+}
+static unsigned stream_grid(size_t D) {
+  return (unsigned)std::min<size_t>((D + BS - 1) / BS, 2048);
+}
+void launch_broyden_diff(hipStream_t st, size_t D, const double* xout, const double* x, double* Fprev,
+                         double* xprev, double* hF, double* hx, int L, int s, unsigned valid, int have_prev,
+                         double* dots) {
+  hipLaunchKernelGGL(k_broyden_diff, dim3(stream_grid(D)), dim3(BS), 0, st, D, xout, x, Fprev, xprev, hF, hx, L, s,
+                     valid, have_prev, dots);
+}
+void launch_broyden_update(hipStream_t st, size_t D, double* x, const double* F, const double* hF,
+                           const double* hx, int L, unsigned valid, const BroydenCoef& g, double alpha,
+                           size_t nclamp) {
+  hipLaunchKernelGGL(k_broyden_update, dim3(stream_grid(D)), dim3(BS), 0, st, D, x, F, hF, hx, L, valid, g, alpha,
+                     nclamp);
 }
 
 }  // namespace nhf

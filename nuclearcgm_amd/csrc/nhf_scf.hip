@@ -21,6 +21,7 @@ namespace nhf {
 constexpr double HBAR = 197.327053;
 constexpr double MNUC = 938.91869;
 constexpr double PI_ = 3.14159265358979323846;
+constexpr size_t DENS_PLANES = 24;
 
 static thread_local hipError_t g_last_hip = hipSuccess;
 
@@ -85,8 +86,22 @@ struct Problem {
   int prec = NHF_PREC_FP64;
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
-  // densities and fields (FP64)
+  // densities and fields (FP64).  The mixed densities rho, tau, J (9
+  // planes), div J of both species are one contiguous vector of
+  // DENS_PLANES * N3 doubles (the vector the Broyden mixer works on):
+  // rho_n, rho_p, tau_n, tau_p, J_n, J_p, divJ_n, divJ_p.
+  double *dens, *dout;
   double *rho[2], *tau[2], *J9[2], *divJ[2], *grho[2], *lrho[2], *Jv[2], *UCdir, *UCoul;
+  double *rho_o[2], *tau_o[2], *J9_o[2], *divJ_o[2];  // unmixed output densities (Broyden)
+  // Broyden mixer: previous residual and input, history of differences
+  double *bFprev = nullptr, *bxprev = nullptr, *bhF = nullptr, *bhx = nullptr, *bdots = nullptr;
+  struct {
+    int L = 0, next = 0;
+    unsigned valid = 0;
+    bool have_prev = false;
+    double alpha = 0.5, w0 = 0.01;
+    std::vector<double> G, nrm;  // raw <dF_i, dF_j>, |dF_i|
+  } bro;
   double* fld64[2];
   float* fld32[2];
   bool fields_set[2] = {false, false};
@@ -162,11 +177,30 @@ static int setup_problem(Problem& P) {
   P.prec = c.precision;
   P.massCorr = c.com_corr ? MNUC * (double)c.A / (double)(c.A - 1) : MNUC;
   const size_t N3 = P.N3;
+  CHK(P.alloc(P.dens, DENS_PLANES * N3));
+  CHK(P.alloc(P.dout, DENS_PLANES * N3));
   for (int q = 0; q < 2; ++q) {
-    CHK(P.alloc(P.rho[q], N3));
-    CHK(P.alloc(P.tau[q], N3));
-    CHK(P.alloc(P.J9[q], 9 * N3));
-    CHK(P.alloc(P.divJ[q], N3));
+    P.rho[q] = P.dens + q * N3;
+    P.tau[q] = P.dens + (2 + q) * N3;
+    P.J9[q] = P.dens + (4 + 9 * q) * N3;
+    P.divJ[q] = P.dens + (22 + q) * N3;
+    P.rho_o[q] = P.dout + q * N3;
+    P.tau_o[q] = P.dout + (2 + q) * N3;
+    P.J9_o[q] = P.dout + (4 + 9 * q) * N3;
+    P.divJ_o[q] = P.dout + (22 + q) * N3;
+  }
+  if (c.mixing == NHF_MIX_BROYDEN) {
+    const int L = c.mix_hist > 0 ? c.mix_hist : 8;
+    const size_t D = DENS_PLANES * N3;
+    P.bro.L = L;
+    P.bro.alpha = c.mix_alpha > 0 ? c.mix_alpha : 0.5;
+    CHK(P.alloc(P.bFprev, D));
+    CHK(P.alloc(P.bxprev, D));
+    CHK(P.alloc(P.bhF, (size_t)L * D));
+    CHK(P.alloc(P.bhx, (size_t)L * D));
+    CHK(P.alloc(P.bdots, 2 * BROYDEN_MAX + 3));
+  }
+  for (int q = 0; q < 2; ++q) {
     CHK(P.alloc(P.grho[q], 3 * N3));
     CHK(P.alloc(P.lrho[q], N3));
     CHK(P.alloc(P.Jv[q], 3 * N3));
@@ -665,6 +699,87 @@ static int bcs_species(Problem& P, Work& W, int q, const Sp<R>* psi, const std::
   return NHF_OK;
 }
 
+// Modified Broyden mixing of the density vector x (D. D. Johnson, Phys.
+// Rev. B 38, 12807 (1988), with unit weights w_n and w0 = 0.01): with the
+// residual F = x_out - x and the normalised differences dF_n, dx_n of the
+// last L steps,
+//   x <- x + alpha F - sum_n gamma_n (alpha dF_n + dx_n),
+//   gamma = (w0^2 I + A)^-1 c,  A_kn = <dF_k, dF_n>,  c_k = <dF_k, F>.
+// The first step after the start is linear with alpha.  Replaces the linear
+// mixing of iteration_data.cpp:928-995 when NHF_MIX_BROYDEN is selected;
+// the fixed point x = x_out is the same.
+static int broyden_step(Problem& P) {
+  hipStream_t st = P.st;
+  auto& B = P.bro;
+  const size_t D = DENS_PLANES * P.N3;
+  const int L = B.L;
+  if (B.G.empty()) {
+    B.G.assign((size_t)L * L, 0.0);
+    B.nrm.assign(L, 0.0);
+  }
+  const int s = B.next;
+  const unsigned others = B.valid & ~(1u << s);
+  HIPCHK(hipMemsetAsync(P.bdots, 0, sizeof(double) * (2 * BROYDEN_MAX + 3), st));
+  launch_broyden_diff(st, D, P.dout, P.dens, P.bFprev, P.bxprev, P.bhF, P.bhx, L, s, others, B.have_prev ? 1 : 0,
+                      P.bdots);
+  double dots[2 * BROYDEN_MAX + 3];
+  HIPCHK(hipMemcpyAsync(dots, P.bdots, sizeof(dots), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  BroydenCoef g{};
+  unsigned valid = others;
+  if (B.have_prev && dots[2 * BROYDEN_MAX] > 0.0 && std::isfinite(dots[2 * BROYDEN_MAX])) {
+    valid |= 1u << s;
+    B.nrm[s] = std::sqrt(dots[2 * BROYDEN_MAX]);
+    B.G[(size_t)s * L + s] = dots[2 * BROYDEN_MAX];
+    for (int i = 0; i < L; ++i)
+      if ((others >> i) & 1u) B.G[(size_t)s * L + i] = B.G[(size_t)i * L + s] = dots[i];
+    B.next = (s + 1) % L;
+  }
+  std::vector<int> idx;
+  for (int i = 0; i < L; ++i)
+    if ((valid >> i) & 1u) idx.push_back(i);
+  const int m = (int)idx.size();
+  if (m > 0) {
+    // beta = (w0^2 I + A)^-1 on the normalised differences; gamma = beta c
+    std::vector<double> A((size_t)m * m), cvec(m);
+    for (int a = 0; a < m; ++a) {
+      const int i = idx[a];
+      cvec[a] = (i == s ? dots[2 * BROYDEN_MAX + 1] : dots[BROYDEN_MAX + i]) / B.nrm[i];
+      for (int b = 0; b < m; ++b) {
+        const int j = idx[b];
+        A[(size_t)a * m + b] = B.G[(size_t)i * L + j] / (B.nrm[i] * B.nrm[j]) + (a == b ? B.w0 * B.w0 : 0.0);
+      }
+    }
+    // Gaussian elimination with partial pivoting (m <= 16)
+    std::vector<double> y = cvec;
+    for (int col = 0; col < m; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < m; ++r)
+        if (std::abs(A[(size_t)r * m + col]) > std::abs(A[(size_t)piv * m + col])) piv = r;
+      if (A[(size_t)piv * m + col] == 0.0) return NHF_ESOLVER;
+      if (piv != col) {
+        for (int k = 0; k < m; ++k) std::swap(A[(size_t)col * m + k], A[(size_t)piv * m + k]);
+        std::swap(y[col], y[piv]);
+      }
+      for (int r = col + 1; r < m; ++r) {
+        const double f = A[(size_t)r * m + col] / A[(size_t)col * m + col];
+        for (int k = col; k < m; ++k) A[(size_t)r * m + k] -= f * A[(size_t)col * m + k];
+        y[r] -= f * y[col];
+      }
+    }
+    for (int r = m - 1; r >= 0; --r) {
+      double v = y[r];
+      for (int k = r + 1; k < m; ++k) v -= A[(size_t)r * m + k] * y[k];
+      y[r] = v / A[(size_t)r * m + r];
+    }
+    for (int a = 0; a < m; ++a) g.g[idx[a]] = y[a] / B.nrm[idx[a]];
+  }
+  launch_broyden_update(st, D, P.dens, P.bFprev, P.bhF, P.bhx, L, valid, g, B.alpha, 4 * P.N3);
+  B.valid = valid;
+  B.have_prev = true;
+  return NHF_OK;
+}
+
 // IterationData::updateQuantities (iteration_data.cpp:852-1125), common
 // part: occupations (solvePairingBCS, :384-465), densities with linear
 // mixing, density moments and the constraint field coefficients.  The
@@ -711,9 +826,16 @@ static int update_common(Problem& P, Sp<R>* const psi[2], const std::vector<doub
   for (int q = 0; q < 2; ++q)
     HIPCHK(hipMemcpyAsync(P.occ[q], P.bcs[q].v2.data(), sizeof(double) * P.M[q], hipMemcpyHostToDevice, st));
   const int first = P.have ? 0 : 1;
-  for (int q = 0; q < 2; ++q)
-    launch_densities<R>(st, psi[q], P.N3, P.M[q], P.occ[q], P.n, P.h, c.density_mix, first, P.rho[q], P.tau[q],
-                        P.J9[q], P.divJ[q]);
+  if (c.mixing == NHF_MIX_BROYDEN && P.have) {
+    for (int q = 0; q < 2; ++q)
+      launch_densities<R>(st, psi[q], P.N3, P.M[q], P.occ[q], P.n, P.h, 1.0, 1, P.rho_o[q], P.tau_o[q], P.J9_o[q],
+                          P.divJ_o[q]);
+    CHK(broyden_step(P));
+  } else {
+    for (int q = 0; q < 2; ++q)
+      launch_densities<R>(st, psi[q], P.N3, P.M[q], P.occ[q], P.n, P.h, c.density_mix, first, P.rho[q], P.tau[q],
+                          P.J9[q], P.divJ[q]);
+  }
   int kinds[NHF_MAX_CON];
   for (size_t i = 0; i < P.cons.size(); ++i) kinds[i] = P.cons[i].c.kind;
   HIPCHK(hipMemsetAsync(P.mom, 0, sizeof(double) * (NHF_MAX_CON + 2), st));
@@ -1085,6 +1207,10 @@ int nhf_problem(void* ctx, const nhf_config* cfg, int* pid) {
   } else if (cfg->pairing_type != NHF_PAIR_NONE) {
     return NHF_EINVAL;  // HFB is not available in this build
   }
+  if ((cfg->mixing != NHF_MIX_LINEAR && cfg->mixing != NHF_MIX_BROYDEN) || cfg->mix_hist < 0 ||
+      cfg->mix_hist > NHF_MAX_HIST || cfg->mix_alpha < 0 || (cfg->wstep != NHF_W_JACOBI_CG && cfg->wstep != NHF_W_LEARNED) ||
+      cfg->wstep_iters < 0)
+    return NHF_EINVAL;
   Ctx* c = (Ctx*)ctx;
   HIPCHK(hipSetDevice(c->device));
   auto P = std::make_unique<Problem>();
@@ -1232,6 +1358,9 @@ int nhf_scf_run(void* ctx, int pid, double* psi_n, double* psi_p, double* eps_n,
   CHK(upload_states(*P, psi_p, P->psi[1][0], P->M[1]));
   // a run starts from scratch: no mixing history, Coulomb warm start zero
   P->have = false;
+  P->bro.next = 0;
+  P->bro.valid = 0;
+  P->bro.have_prev = false;
   P->energyDiff = 1.0;
   P->cons.clear();
   for (int i = 0; i < P->cfg.n_con; ++i) {

@@ -16,6 +16,15 @@ pytestmark = pytest.mark.gpu
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ca40_oracle.json")))
 
 
+def gold_iterations_to_1kev():
+    """Iterations after which the oracle's (linear-mixing) E_int stays within 1 keV."""
+    inside = np.abs(np.array(GOLD["trace_E_int"]) - GOLD["E_int"]) <= 1e-3
+    k = len(inside)
+    while k > 0 and inside[k - 1]:
+        k -= 1
+    return k + 1
+
+
 @pytest.fixture(scope="module")
 def ctx():
     from nuclearcgm_amd import hip
@@ -25,28 +34,43 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("prec,tol_kev", [(C.PREC_FP64, 0.01), (C.PREC_MIXED, 1.0)])
-def test_ca40_matches_oracle(ctx, prec, tol_kev):
+@pytest.mark.parametrize("prec,mixing,tol_kev", [(C.PREC_FP64, C.MIX_LINEAR, 0.01), (C.PREC_MIXED, C.MIX_LINEAR, 1.0),
+                                              (C.PREC_FP64, C.MIX_BROYDEN, 0.05), (C.PREC_MIXED, C.MIX_BROYDEN, 1.0)])
+def test_ca40_matches_oracle(ctx, prec, mixing, tol_kev):
     # FP64 path: converged E_tot within 10 eV of the oracle (both stop at
     # |dE| < 1e-6 MeV; the device reductions use atomics, so runs differ in
     # the last bits); accelerated FP32 path: within the 1 keV target.
+    # Broyden mixing changes the trajectory, not the fixed point; its late
+    # steps are small, so the stop rule |E_k - E_k-1| < 1e-6 MeV fires
+    # further from the fixed point than with linear mixing: 50 eV for FP64.
     cfg = C.workload("ca40", precision=prec)
     cfg.max_iter = 600
+    cfg.mixing = mixing
     g = bench.make_guess(cfg)
     pn, pp, en, ep, res, trace = ctx.problem(cfg).scf_run(*g)
     s = res.summary()
     dE = (s["E_int"] - GOLD["E_int"]) * 1e3
-    print(f"prec {prec}: E {s['E_int']:.7f} dE {dE:.4f} keV, its {s['iterations']}, {s['seconds']:.3f} s")
+    print(f"prec {prec} mixing {mixing}: E {s['E_int']:.7f} dE {dE:.4f} keV, its {s['iterations']}, "
+          f"{s['seconds']:.3f} s")
     assert s["converged"]
     assert abs(dE) < tol_kev
     # the iteration-0 energy depends only on the guess (stored in FP32 on the
     # accelerated path: densities to ~1e-7 relative)
     assert abs(trace[0, 0] - GOLD["trace_E_int"][0]) < (1e-8 if prec == C.PREC_FP64 else 1e-5)
+    if mixing == C.MIX_BROYDEN:  # the acceleration is real: fewer iterations to 1 keV
+        _, k = bench.time_to_window(trace, GOLD["E_int"])
+        assert 0 < k < 0.7 * gold_iterations_to_1kev()
     assert np.abs(en - np.array(GOLD["eps_n"])).max() < (1e-4 if prec == C.PREC_FP64 else 1e-3)
     assert np.abs(ep - np.array(GOLD["eps_p"])).max() < (1e-4 if prec == C.PREC_FP64 else 1e-3)
     assert abs(s["rms_n"] - GOLD["rms_n"]) < 1e-5 and abs(s["rms_p"] - GOLD["rms_p"]) < 1e-5
     assert abs(s["beta2"] - GOLD["beta2"]) < 1e-5
     # BCS chemical potentials (MeV): the runs stop on |dE| < 1e-6 MeV, which
     # leaves lambda converged only to ~1e-4 (FP64) and ~1e-3 (FP32)
+    # (same trajectory as the oracle).  Ca-40 is doubly magic: the BCS gap
+    # nearly collapses and the algebraic lambda update fixes lambda only
+    # weakly, so a different trajectory (Broyden) ends ~1e-2 MeV away with
+    # the same energies, states and occupations.
     ltol = 1e-3 if prec == C.PREC_FP64 else 3e-3
+    if mixing == C.MIX_BROYDEN:
+        ltol = 5e-2
     assert abs(s["lambda_n"] - GOLD["lambda_n"]) < ltol and abs(s["lambda_p"] - GOLD["lambda_p"]) < ltol
