@@ -124,7 +124,7 @@ typedef struct nhf_config {
   int mix_hist;          /* Broyden history length, 1..NHF_MAX_HIST (0 -> 8) */
   double mix_alpha;      /* Broyden linear part (0 -> 0.5) */
   int wstep;             /* NHF_W_JACOBI_CG (reference) | NHF_W_LEARNED */
-  int wstep_iters;       /* learned W-step: flexible-PCG iterations (0 -> 1) */
+  int wstep_iters;       /* learned W-block: minimal-residual steps, 1..8 (0 -> 1) */
 } nhf_config;
 
 /* Density mixing.  The reference mixes linearly (iteration_data.cpp:928-995);
@@ -139,9 +139,11 @@ typedef struct nhf_config {
 
 /* The GCG W-block.  NHF_W_JACOBI_CG is the reference's Jacobi-PCG solve of
  * (H + sigma) w = (lambda + sigma) x (solver.cpp:727-745); NHF_W_LEARNED
- * replaces the Jacobi preconditioner by the context's learned 3D
- * convolutional preconditioner (nhf_lcp_*) and runs wstep_iters iterations,
- * falling back to NHF_W_JACOBI_CG for a species whose residual stalls. */
+ * runs wstep_iters preconditioned minimal-residual steps on the same system
+ * with the context's learned 3D convolutional preconditioner (nhf_lcp_*).
+ * When the mean residual reduction |r_k|/|r_0| of such a call exceeds 0.9
+ * (a stall), that call falls back to the Jacobi-PCG W-block and so do the
+ * next two calls of the species. */
 #define NHF_W_JACOBI_CG 0
 #define NHF_W_LEARNED 1
 
@@ -278,6 +280,46 @@ int nhf_cholqr_coeffs(int k, const double *G, double drop_rel, int *keep, double
 /* Skyrme (t,x,W0,sigma) -> EDF coupling constants (EDF.cpp:76-98).
  * skyrme: t0 t1 t2 t3 x0 x1 x2 x3 W0 W0' te to sigma. */
 void nhf_edf_from_skyrme(const double skyrme[13], double edf[NHF_EDF_N]);
+
+/* ---- Learned convolutional preconditioner of the GCG W-block.
+ * No reference counterpart: it stands in for the Jacobi-PCG of
+ * solver.cpp:727-745 when a problem has wstep = NHF_W_LEARNED.  One net per
+ * context, shared by its problems.  The net works in lattice units: input
+ * per point (r/s: 4 real spinor channels, kappa = h^2 (U + sigma)/f, three
+ * zero pad channels), output c/(s h^2/f) (4 channels), s = rms(r); four
+ * 3x3x3 convolutions 5 -> 16 -> 16 -> 16 -> 4, LeakyReLU(0.1), zero
+ * padding.  Parameters (FP32): per layer W[tap][ci][co] then b[co]. */
+int nhf_lcp_num_params(void);
+/* He-normal initialisation from a seed (splitmix64 + Box-Muller). */
+int nhf_lcp_init(void *ctx, unsigned long long seed);
+int nhf_lcp_set_params(void *ctx, const float *w, int count);
+int nhf_lcp_get_params(void *ctx, float *w, int count);
+/* FP32 forward on host data: in8 = batch x n^3 x 8, out4 = batch x n^3 x 4
+ * (batch <= 16). */
+int nhf_lcp_forward(void *ctx, int n, int batch, const float *in8, float *out4);
+/* Loss sum_b |A (y_b - t_b)|^2 / (batch |A t_b|^2) and its parameter
+ * gradient (nhf_lcp_num_params floats), computed by the training kernels;
+ * A = -lap4 + kappa in lattice units (13-point 4th-order Laplacian, zero
+ * outside the box, kappa = input channel 4) weighs the error of the
+ * correction by the residual it leaves. */
+int nhf_lcp_loss_grad(void *ctx, int n, int batch, const float *in8, const float *tgt4, double *loss, float *grad);
+/* Inference on host data: z_c = s_c (h^2/f) net(r_c / s_c, kappa) for m
+ * columns of n^3 points x 4 floats (s_c = rms of r_c), kap and hf n^3
+ * each; mode 0 = the BF16 MFMA kernels of the W-block, 1 = FP32 kernels. */
+int nhf_lcp_apply(void *ctx, int n, int m, const float *r, const float *kap, const float *hf, float *z, int mode);
+/* Arm the collection of up to max_pairs (residual, converged correction)
+ * pairs from the plain W-blocks of the context's problems, one GCG call in
+ * `every` per species. */
+int nhf_lcp_collect(void *ctx, int max_pairs, int every);
+int nhf_lcp_num_pairs(void *ctx);
+int nhf_lcp_clear_pairs(void *ctx);
+/* Adam on random batches of the collected pairs (random global phase per
+ * sample); mean loss of the first and last ten steps. */
+int nhf_lcp_train(void *ctx, int steps, int batch, double lr, unsigned long long seed, double *loss_first,
+                  double *loss_last);
+/* Per species q: out[4q + 0] GCG calls, [1] learned W-blocks, [2] of them
+ * fallen back to Jacobi-PCG, [3] mean |r_k|/|r_0| of the learned steps. */
+int nhf_lcp_stats(void *ctx, int pid, int reset, double *out);
 
 #ifdef __cplusplus
 }

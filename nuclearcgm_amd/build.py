@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnhf_hip.so")
-SOURCES = ["nhf_kernels.hip", "nhf_scf.hip", "nhf_dense.cpp", "nhf_pairing.cpp", "nhf_guess.cpp"]
+SOURCES = ["nhf_kernels.hip", "nhf_scf.hip", "nhf_lcp.hip", "nhf_dense.cpp", "nhf_pairing.cpp", "nhf_guess.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

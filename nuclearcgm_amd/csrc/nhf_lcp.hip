@@ -1,0 +1,1038 @@
+// nhf_lcp.hip -- kernels of the learned convolutional preconditioner:
+// FP32 forward / backward-data / backward-weight convolutions (training and
+// inference), the loss, Adam, and the collection of (residual, correction)
+// training pairs from the solver.  Channels-last activations: sample b,
+// point p, channel c at ((b n^3) + p) C + c.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+
+#include "nhf_lcp.h"
+
+namespace nhf {
+
+namespace {
+constexpr int LBS = 256;
+constexpr int BWD_W_BLOCKS = 512;
+
+__device__ __forceinline__ void lcp_decompose(size_t p, int n, int& i, int& j, int& k) {
+  const int nn = n * n;
+  k = (int)(p / (size_t)nn);
+  const int rem = (int)(p - (size_t)k * nn);
+  j = rem / n;
+  i = rem - j * n;
+}
+__device__ __forceinline__ float leaky(float x) { return x > 0.f ? x : LCP_SLOPE * x; }
+
+template <int NV>
+__device__ __forceinline__ void block_sum_add(double (&v)[NV], double* dst) {
+  __shared__ double sh[LBS / 64][NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[wid][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < LBS / 64; ++w) s += sh[w][threadIdx.x];
+    atomicAdd(dst + threadIdx.x, s);
+  }
+}
+
+// one 3x3x3 layer, one thread per output point; weights staged in LDS
+template <int CIr, int CIs, int CO, bool ACT>
+__global__ __launch_bounds__(LBS) void k_fwd(const float* __restrict__ in, size_t in_ss,
+                                             const float* __restrict__ prm, float* __restrict__ out, size_t out_ss,
+                                             int n) {
+  constexpr int NW = LCP_TAPS * CIr * CO;
+  __shared__ float ws[NW + CO];
+  for (int t = threadIdx.x; t < NW + CO; t += LBS) ws[t] = prm[t];
+  __syncthreads();
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  if (p >= N3) return;
+  int i, j, k;
+  lcp_decompose(p, n, i, j, k);
+  const float* __restrict__ src = in + (size_t)blockIdx.y * in_ss;
+  float acc[CO];
+#pragma unroll
+  for (int co = 0; co < CO; ++co) acc[co] = ws[NW + co];
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int kk = k + dz;
+    if (kk < 0 || kk >= n) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int jj = j + dy;
+      if (jj < 0 || jj >= n) continue;
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int ii = i + dx;
+        if (ii < 0 || ii >= n) continue;
+        const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+        const float* __restrict__ x = src + ((size_t)ii + (size_t)n * ((size_t)jj + (size_t)n * kk)) * CIs;
+        float xv[CIr];
+#pragma unroll
+        for (int ci = 0; ci < CIr; ++ci) xv[ci] = x[ci];
+        const float* wt = ws + tap * CIr * CO;
+#pragma unroll
+        for (int ci = 0; ci < CIr; ++ci)
+#pragma unroll
+          for (int co = 0; co < CO; ++co) acc[co] = fmaf(xv[ci], wt[ci * CO + co], acc[co]);
+      }
+    }
+  }
+  float* __restrict__ dst = out + (size_t)blockIdx.y * out_ss + p * CO;
+#pragma unroll
+  for (int co = 0; co < CO; ++co) dst[co] = ACT ? leaky(acc[co]) : acc[co];
+}
+
+// gradient with respect to the layer input, times LeakyReLU' of that input:
+// gin[q][ci] = leaky'(a[q][ci]) sum_tap sum_co gz[q - off(tap)][co] W[tap][ci][co]
+template <int CIr, int CIs, int CO>
+__global__ __launch_bounds__(LBS) void k_bwd_data(const float* __restrict__ gz, size_t gz_ss,
+                                                  const float* __restrict__ prm, const float* __restrict__ a,
+                                                  size_t a_ss, float* __restrict__ gin, size_t gin_ss, int n) {
+  constexpr int NW = LCP_TAPS * CIr * CO;
+  __shared__ float ws[NW];
+  for (int t = threadIdx.x; t < NW; t += LBS) ws[t] = prm[t];
+  __syncthreads();
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  if (p >= N3) return;
+  int i, j, k;
+  lcp_decompose(p, n, i, j, k);
+  const float* __restrict__ src = gz + (size_t)blockIdx.y * gz_ss;
+  float acc[CIr];
+#pragma unroll
+  for (int ci = 0; ci < CIr; ++ci) acc[ci] = 0.f;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int kk = k - dz;
+    if (kk < 0 || kk >= n) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int jj = j - dy;
+      if (jj < 0 || jj >= n) continue;
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int ii = i - dx;
+        if (ii < 0 || ii >= n) continue;
+        const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+        const float* __restrict__ g = src + ((size_t)ii + (size_t)n * ((size_t)jj + (size_t)n * kk)) * CO;
+        float gv[CO];
+#pragma unroll
+        for (int co = 0; co < CO; ++co) gv[co] = g[co];
+        const float* wt = ws + tap * CIr * CO;
+#pragma unroll
+        for (int ci = 0; ci < CIr; ++ci)
+#pragma unroll
+          for (int co = 0; co < CO; ++co) acc[ci] = fmaf(gv[co], wt[ci * CO + co], acc[ci]);
+      }
+    }
+  }
+  const float* __restrict__ av = a + (size_t)blockIdx.y * a_ss + p * CIs;
+  float* __restrict__ dst = gin + (size_t)blockIdx.y * gin_ss + p * CIs;
+#pragma unroll
+  for (int ci = 0; ci < CIr; ++ci) dst[ci] = acc[ci] * (av[ci] > 0.f ? 1.f : LCP_SLOPE);
+}
+
+// weight gradient partial sums: thread (ci, co), the block walks points
+// e = blockIdx.x, blockIdx.x + gridDim.x, ...; part[block][tap ci co | co]
+template <int CIr, int CIs, int CO>
+__global__ __launch_bounds__(LBS) void k_bwd_w(const float* __restrict__ in, size_t in_ss,
+                                               const float* __restrict__ gz, size_t gz_ss, int n, int nb,
+                                               float* __restrict__ part) {
+  constexpr int NW = LCP_TAPS * CIr * CO;
+  static_assert(CIr * CO <= LBS, "one thread per (ci, co)");
+  const int t = threadIdx.x;
+  const bool on = t < CIr * CO;
+  const int ci = on ? t / CO : 0, co = on ? t % CO : 0;
+  float acc[LCP_TAPS];
+#pragma unroll
+  for (int q = 0; q < LCP_TAPS; ++q) acc[q] = 0.f;
+  float accb = 0.f;
+  const size_t N3 = (size_t)n * n * n, total = N3 * (size_t)nb;
+  for (size_t e = blockIdx.x; e < total; e += gridDim.x) {
+    const size_t b = e / N3, p = e - b * N3;
+    int i, j, k;
+    lcp_decompose(p, n, i, j, k);
+    const float g = gz[b * gz_ss + p * CO + co];
+    accb += g;
+    const float* __restrict__ src = in + b * in_ss;
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int ii = i + dx, jj = j + dy, kk = k + dz;
+          if (ii < 0 || ii >= n || jj < 0 || jj >= n || kk < 0 || kk >= n) continue;
+          const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+          acc[tap] = fmaf(src[((size_t)ii + (size_t)n * ((size_t)jj + (size_t)n * kk)) * CIs + ci], g, acc[tap]);
+        }
+  }
+  if (!on) return;
+  float* dst = part + (size_t)blockIdx.x * (NW + CO);
+#pragma unroll
+  for (int q = 0; q < LCP_TAPS; ++q) dst[(q * CIr + ci) * CO + co] = acc[q];
+  if (ci == 0) dst[NW + co] = accb;
+}
+
+__global__ __launch_bounds__(LBS) void k_reduce_part(const float* __restrict__ part, int nblk, int cnt,
+                                                     float* __restrict__ g) {
+  const int i = blockIdx.x * LBS + threadIdx.x;
+  if (i >= cnt) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[(size_t)b * cnt + i];
+  g[i] = (float)s;
+}
+
+struct LcpWts {
+  double w[16];
+};
+// The loss weighs the error of the correction by the model operator of
+// (h^2/f)(H + sigma) in lattice units, A = -lap4_lattice + kappa (13-point
+// 4th-order Laplacian, zero outside the box, kappa = input channel 4), so
+// that it measures the residual the correction leaves, where a
+// preconditioner's quality shows:  L = sum_b w_b |A (y_b - t_b)|^2.
+__device__ __forceinline__ void aop_point(const float* __restrict__ y, const float* __restrict__ t,
+                                          const float* __restrict__ x0, int n, int i, int j, int k, size_t p,
+                                          float (&out)[4]) {
+  const int pos[3] = {i, j, k};
+  const size_t stride[3] = {1, (size_t)n, (size_t)n * n};
+  auto val = [&](size_t q, int c) { return (y ? y[q * 4 + c] : 0.f) - t[q * 4 + c]; };
+  const float kap = x0[p * LCP_IN_STRIDE + 4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[c] = (7.5f + kap) * val(p, c);
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+    for (int d = -2; d <= 2; ++d) {
+      if (d == 0) continue;
+      const int e = pos[ax] + d;
+      if (e < 0 || e >= n) continue;
+      const float cf = (d == 1 || d == -1) ? (-16.f / 12.f) : (1.f / 12.f);
+      const size_t q = p + (ptrdiff_t)d * (ptrdiff_t)stride[ax];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) out[c] += cf * val(q, c);
+    }
+}
+// e = A (y - t) (y may be null: e = -A t); per_sample: acc[b] += |e_b|^2,
+// else acc[0] += sum_b w_b |e_b|^2.  e_out may be null.
+__global__ __launch_bounds__(LBS) void k_aop_norm(const float* __restrict__ y, const float* __restrict__ t,
+                                                  const float* __restrict__ x0, size_t N3, int n, LcpWts wt,
+                                                  float* __restrict__ e_out, double* __restrict__ acc,
+                                                  int per_sample) {
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  const int b = blockIdx.y;
+  double l[1] = {0.0};
+  if (p < N3) {
+    int i, j, k;
+    lcp_decompose(p, n, i, j, k);
+    float e[4];
+    aop_point(y ? y + (size_t)b * N3 * 4 : nullptr, t + (size_t)b * N3 * 4, x0 + (size_t)b * N3 * LCP_IN_STRIDE, n,
+              i, j, k, p, e);
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      s += (double)e[c] * e[c];
+      if (e_out) e_out[((size_t)b * N3 + p) * 4 + c] = e[c];
+    }
+    l[0] = per_sample ? s : wt.w[b] * s;
+  }
+  block_sum_add<1>(l, acc + (per_sample ? b : 0));
+}
+// gy = 2 w_b A e_b  (A is symmetric)
+__global__ __launch_bounds__(LBS) void k_aop_grad(const float* __restrict__ e, const float* __restrict__ x0,
+                                                  size_t N3, int n, LcpWts wt, float* __restrict__ gy) {
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  const int b = blockIdx.y;
+  if (p >= N3) return;
+  int i, j, k;
+  lcp_decompose(p, n, i, j, k);
+  float o[4];
+  // A e = -A (0 - e)
+  aop_point(nullptr, e + (size_t)b * N3 * 4, x0 + (size_t)b * N3 * LCP_IN_STRIDE, n, i, j, k, p, o);
+  const float f = (float)(-2.0 * wt.w[b]);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) gy[((size_t)b * N3 + p) * 4 + c] = f * o[c];
+}
+
+__global__ __launch_bounds__(LBS) void k_adam(float* __restrict__ w, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, int cnt, float lr,
+                                              float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * LBS + threadIdx.x;
+  if (i >= cnt) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// ---- inference: residual columns -> first layer, last layer -> correction
+template <typename R>
+__global__ __launch_bounds__(LBS) void k_in(const Sp<R>* __restrict__ r, size_t ld, const int* __restrict__ clist,
+                                            const double* __restrict__ rn2, const float* __restrict__ kap,
+                                            const float* __restrict__ prm, float* __restrict__ out, int n) {
+  constexpr int CIr = 5, CO = LCP_HID, NW = LCP_TAPS * CIr * CO;
+  __shared__ float ws[NW + CO];
+  for (int t = threadIdx.x; t < NW + CO; t += LBS) ws[t] = prm[t];
+  __syncthreads();
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  if (p >= N3) return;
+  const int c = clist ? clist[blockIdx.y] : (int)blockIdx.y;
+  const double s2 = rn2[c] / (4.0 * (double)N3);
+  const float inv = s2 > 0.0 ? (float)(1.0 / sqrt(s2)) : 0.f;
+  int i, j, k;
+  lcp_decompose(p, n, i, j, k);
+  const Sp<R>* __restrict__ col = r + (size_t)c * ld;
+  float acc[CO];
+#pragma unroll
+  for (int co = 0; co < CO; ++co) acc[co] = ws[NW + co];
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int kk = k + dz;
+    if (kk < 0 || kk >= n) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int jj = j + dy;
+      if (jj < 0 || jj >= n) continue;
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int ii = i + dx;
+        if (ii < 0 || ii >= n) continue;
+        const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+        const size_t q = (size_t)ii + (size_t)n * ((size_t)jj + (size_t)n * kk);
+        const Sp<R> v = col[q];
+        const float xv[CIr] = {(float)v.ur * inv, (float)v.ui * inv, (float)v.dr * inv, (float)v.di * inv, kap[q]};
+        const float* wt = ws + tap * CIr * CO;
+#pragma unroll
+        for (int ci = 0; ci < CIr; ++ci)
+#pragma unroll
+          for (int co = 0; co < CO; ++co) acc[co] = fmaf(xv[ci], wt[ci * CO + co], acc[co]);
+      }
+    }
+  }
+  float* __restrict__ dst = out + ((size_t)blockIdx.y * N3 + p) * CO;
+#pragma unroll
+  for (int co = 0; co < CO; ++co) dst[co] = leaky(acc[co]);
+}
+
+template <typename R>
+__global__ __launch_bounds__(LBS) void k_out(const float* __restrict__ in, const float* __restrict__ prm,
+                                             Sp<R>* __restrict__ z, size_t ld, const int* __restrict__ clist,
+                                             const double* __restrict__ rn2, const float* __restrict__ hf, int n) {
+  constexpr int CIr = LCP_HID, CO = 4, NW = LCP_TAPS * CIr * CO;
+  __shared__ float ws[NW + CO];
+  for (int t = threadIdx.x; t < NW + CO; t += LBS) ws[t] = prm[t];
+  __syncthreads();
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  if (p >= N3) return;
+  const int c = clist ? clist[blockIdx.y] : (int)blockIdx.y;
+  const double s = sqrt(rn2[c] / (4.0 * (double)N3));
+  int i, j, k;
+  lcp_decompose(p, n, i, j, k);
+  const float* __restrict__ src = in + (size_t)blockIdx.y * N3 * CIr;
+  float acc[CO];
+#pragma unroll
+  for (int co = 0; co < CO; ++co) acc[co] = ws[NW + co];
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int kk = k + dz;
+    if (kk < 0 || kk >= n) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int jj = j + dy;
+      if (jj < 0 || jj >= n) continue;
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int ii = i + dx;
+        if (ii < 0 || ii >= n) continue;
+        const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+        const float* __restrict__ x = src + ((size_t)ii + (size_t)n * ((size_t)jj + (size_t)n * kk)) * CIr;
+        float xv[CIr];
+#pragma unroll
+        for (int ci = 0; ci < CIr; ++ci) xv[ci] = x[ci];
+        const float* wt = ws + tap * CIr * CO;
+#pragma unroll
+        for (int ci = 0; ci < CIr; ++ci)
+#pragma unroll
+          for (int co = 0; co < CO; ++co) acc[co] = fmaf(xv[ci], wt[ci * CO + co], acc[co]);
+      }
+    }
+  }
+  const float sc = (float)s * hf[p];
+  z[(size_t)c * ld + p] = Sp<R>{(R)(acc[0] * sc), (R)(acc[1] * sc), (R)(acc[2] * sc), (R)(acc[3] * sc)};
+}
+
+template <typename R>
+__global__ __launch_bounds__(LBS) void k_planes(const R* __restrict__ fld, size_t N3, double h2, double shift,
+                                                float* __restrict__ kap, float* __restrict__ hf) {
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  if (p >= N3) return;
+  const double f = (double)fld[(size_t)NFLD * p + 0], U = (double)fld[(size_t)NFLD * p + 4];
+  kap[p] = (float)(h2 * (U + shift) / f);
+  hf[p] = (float)(h2 / f);
+}
+
+// ---- training pairs
+template <typename R>
+__global__ __launch_bounds__(LBS) void k_pair_rn2(const Sp<R>* __restrict__ T, const Sp<R>* __restrict__ X,
+                                                  const double* __restrict__ ls, size_t ld, size_t N3,
+                                                  double* __restrict__ rn2) {
+  const int c = blockIdx.y;
+  double v[1] = {0.0};
+  for (size_t p = (size_t)blockIdx.x * LBS + threadIdx.x; p < N3; p += (size_t)gridDim.x * LBS) {
+    const Sp<R> t = T[(size_t)c * ld + p], x = X[(size_t)c * ld + p];
+    const double l = ls[c];
+    const double a = t.ur - l * x.ur, b = t.ui - l * x.ui, d = t.dr - l * x.dr, e = t.di - l * x.di;
+    v[0] += a * a + b * b + d * d + e * e;
+  }
+  block_sum_add<1>(v, rn2 + c);
+}
+template <typename R>
+__global__ __launch_bounds__(LBS) void k_pair_store(const Sp<R>* __restrict__ T, const Sp<R>* __restrict__ X,
+                                                    const Sp<R>* __restrict__ W, const double* __restrict__ ls,
+                                                    size_t ld, size_t N3, const double* __restrict__ rn2,
+                                                    const float* __restrict__ kap, const float* __restrict__ hf,
+                                                    float* const* __restrict__ dst) {
+  const int c = blockIdx.y;
+  float* __restrict__ in8 = dst[c];
+  float* __restrict__ tg = in8 + N3 * LCP_IN_STRIDE;
+  const double inv = 1.0 / sqrt(rn2[c] / (4.0 * (double)N3));
+  for (size_t p = (size_t)blockIdx.x * LBS + threadIdx.x; p < N3; p += (size_t)gridDim.x * LBS) {
+    const size_t o = (size_t)c * ld + p;
+    const Sp<R> t = T[o], x = X[o], w = W[o];
+    const double l = ls[c];
+    float* a = in8 + p * LCP_IN_STRIDE;
+    a[0] = (float)((t.ur - l * x.ur) * inv);
+    a[1] = (float)((t.ui - l * x.ui) * inv);
+    a[2] = (float)((t.dr - l * x.dr) * inv);
+    a[3] = (float)((t.di - l * x.di) * inv);
+    a[4] = kap[p];
+    a[5] = a[6] = a[7] = 0.f;
+    const double sc = inv / (double)hf[p];
+    const float g0 = (float)((x.ur - w.ur) * sc), g1 = (float)((x.ui - w.ui) * sc),
+                g2 = (float)((x.dr - w.dr) * sc), g3 = (float)((x.di - w.di) * sc);
+    float* b = tg + p * 4;
+    b[0] = g0;
+    b[1] = g1;
+    b[2] = g2;
+    b[3] = g3;
+  }
+}
+
+
+// ---- BF16 MFMA inference (v_mfma_f32_16x16x16_bf16).  A workgroup owns a
+// brick of 8 x 8 x 4 lattice points; the brick and its one-point halo
+// (10 x 10 x 6 points, zeros outside the box) are staged once in LDS, so
+// every activation is read from HBM/L2 once per brick instead of once per
+// tap.  A wavefront computes tiles of 16 points (8 in x, 2 in y) x 16 output
+// channels; K runs over taps x input channels.  Operand layout of the
+// 16x16x16 MFMA: lane l holds A[l % 16][4 (l / 16) + r], B[4 (l / 16) + r][l % 16]
+// and D[4 (l / 16) + r][l % 16], r = 0..3.  Activations between layers are
+// BF16 (n^3 x 16 per column); weights are per-lane B fragments prepared from
+// the FP32 master copy (k_make_frags).
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int FRAG_L0 = 0, FRAG_L1 = 14, FRAG_L2 = 41, FRAG_L3 = 68, FRAG_N = 95;
+constexpr int BX = 8, BY = 8, BZ = 4, HX = BX + 2, HY = BY + 2, HZ = BZ + 2, HPTS = HX * HY * HZ;
+constexpr int S16 = 20;  // LDS stride (bf16) of a 16-channel point: conflict-free 8-byte reads
+constexpr int S8 = 12;   // LDS stride of the 8-channel input point
+
+__device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest even
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// fragments: layer 0 has 14 MFMAs over 2 taps x 8 input channels (residual
+// 0-3, kappa 4, zero 5-7), layers 1-3 one per tap (layer 3 padded to 16
+// output channels with zeros)
+__global__ __launch_bounds__(LBS) void k_make_frags(const float* __restrict__ w, bf16x4* __restrict__ frags) {
+  const int id = blockIdx.x * LBS + threadIdx.x;
+  if (id >= FRAG_N * 64) return;
+  const int m = id / 64, lane = id % 64, g = lane / 16, col = lane % 16;
+  float v[4];
+  for (int r = 0; r < 4; ++r) {
+    float x = 0.f;
+    if (m < FRAG_L1) {
+      const int t = 2 * m + g / 2, ci = 4 * (g % 2) + r;
+      if (t < LCP_TAPS && ci < 5) x = w[lcp_layer_offset(0) + (t * 5 + ci) * 16 + col];
+    } else if (m < FRAG_L3) {
+      const int l = m < FRAG_L2 ? 1 : 2, t = m - (l == 1 ? FRAG_L1 : FRAG_L2);
+      x = w[lcp_layer_offset(l) + (t * 16 + 4 * g + r) * 16 + col];
+    } else {
+      const int t = m - FRAG_L3;
+      if (col < 4) x = w[lcp_layer_offset(3) + (t * 16 + 4 * g + r) * 4 + col];
+    }
+    v[r] = x;
+  }
+  frags[id] = bf16x4{(short)f2bf(v[0]), (short)f2bf(v[1]), (short)f2bf(v[2]), (short)f2bf(v[3])};
+}
+
+struct Brick {
+  int x0, y0, z0;
+};
+__device__ __forceinline__ Brick brick_of(int n) {
+  const int nbx = (n + BX - 1) / BX, nby = (n + BY - 1) / BY;
+  const int b = blockIdx.x;
+  return Brick{(b % nbx) * BX, ((b / nbx) % nby) * BY, (b / (nbx * nby)) * BZ};
+}
+__host__ __device__ inline unsigned bricks_of(int n) {
+  return (unsigned)(((n + BX - 1) / BX) * ((n + BY - 1) / BY) * ((n + BZ - 1) / BZ));
+}
+// halo point h -> lattice coordinates; false outside the box
+__device__ __forceinline__ bool halo_point(const Brick& B, int h, int n, size_t& q) {
+  const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+  const int x = B.x0 + hx - 1, y = B.y0 + hy - 1, z = B.z0 + hz - 1;
+  if (x < 0 || x >= n || y < 0 || y >= n || z < 0 || z >= n) return false;
+  q = (size_t)x + (size_t)n * ((size_t)y + (size_t)n * z);
+  return true;
+}
+
+// layer 0: residual column (scaled by 1/s) and kappa -> 16 channels
+template <typename R>
+__global__ __launch_bounds__(LBS) void k_m_in(const Sp<R>* __restrict__ r, size_t ld, const int* __restrict__ clist,
+                                              const double* __restrict__ rn2, const float* __restrict__ kap,
+                                              const bf16x4* __restrict__ frags, const float* __restrict__ w,
+                                              unsigned short* __restrict__ out, int n, int rs) {
+  __shared__ __align__(16) unsigned short lds[HPTS * S8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
+  const size_t N3 = (size_t)n * n * n;
+  const int c = clist ? clist[blockIdx.y] : (int)blockIdx.y;
+  const double s2 = rn2[(size_t)c * rs] / (4.0 * (double)N3);
+  const float inv = s2 > 0.0 ? (float)(1.0 / sqrt(s2)) : 0.f;
+  const Sp<R>* __restrict__ src = r + (size_t)c * ld;
+  const Brick Bk = brick_of(n);
+  for (int h = threadIdx.x; h < HPTS; h += LBS) {
+    size_t q;
+    unsigned short v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (halo_point(Bk, h, n, q)) {
+      const Sp<R> x = src[q];
+      v[0] = f2bf((float)x.ur * inv);
+      v[1] = f2bf((float)x.ui * inv);
+      v[2] = f2bf((float)x.dr * inv);
+      v[3] = f2bf((float)x.di * inv);
+      v[4] = f2bf(kap[q]);
+    }
+    bf16x4* d = reinterpret_cast<bf16x4*>(lds + h * S8);
+    d[0] = bf16x4{(short)v[0], (short)v[1], (short)v[2], (short)v[3]};
+    d[1] = bf16x4{(short)v[4], 0, 0, 0};
+  }
+  bf16x4 B[14];
+#pragma unroll
+  for (int m = 0; m < 14; ++m) B[m] = frags[(FRAG_L0 + m) * 64 + lane];
+  const float bs = w[lcp_layer_offset(0) + LCP_TAPS * 5 * 16 + col];
+  __syncthreads();
+  unsigned short* __restrict__ dst = out + (size_t)blockIdx.y * N3 * 16;
+  const int xa = col % 8;
+  for (int yp = 0; yp < BY / 2; ++yp) {
+    const int ya = 2 * yp + col / 8, za = wave;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int m = 0; m < 14; ++m) {
+      const int t = 2 * m + g / 2;
+      bf16x4 a = {0, 0, 0, 0};
+      if (t < LCP_TAPS) {
+        const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // halo offsets 0..2
+        const int h = ((za + dz) * HY + (ya + dy)) * HX + (xa + dx);
+        a = *reinterpret_cast<const bf16x4*>(lds + h * S8 + 4 * (g % 2));
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[m], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int row = 4 * g + rr;
+      const int x = Bk.x0 + row % 8, y = Bk.y0 + 2 * yp + row / 8, z = Bk.z0 + za;
+      if (x < n && y < n && z < n)
+        dst[((size_t)x + (size_t)n * ((size_t)y + (size_t)n * z)) * 16 + col] = f2bf(leaky(acc[rr] + bs));
+    }
+  }
+}
+
+// layers 1-2 (16 -> 16, LeakyReLU) and 3 (16 -> 4, scaled into the
+// correction column)
+template <int LAYER, typename R>
+__global__ __launch_bounds__(LBS) void k_m_hid(const unsigned short* __restrict__ in,
+                                               const bf16x4* __restrict__ frags, const float* __restrict__ w,
+                                               unsigned short* __restrict__ out, Sp<R>* __restrict__ z, size_t ld,
+                                               const int* __restrict__ clist, const double* __restrict__ rn2,
+                                               const float* __restrict__ hf, int n, int rs) {
+  constexpr int F0 = LAYER == 1 ? FRAG_L1 : LAYER == 2 ? FRAG_L2 : FRAG_L3;
+  constexpr int CO = LAYER == 3 ? 4 : 16;
+  __shared__ __align__(16) unsigned short lds[HPTS * S16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
+  const size_t N3 = (size_t)n * n * n;
+  const unsigned short* __restrict__ src = in + (size_t)blockIdx.y * N3 * 16;
+  const Brick Bk = brick_of(n);
+  // two 16-byte halves per halo point
+  for (int e = threadIdx.x; e < 2 * HPTS; e += LBS) {
+    const int h = e >> 1, half = e & 1;
+    size_t q;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (halo_point(Bk, h, n, q)) v = *reinterpret_cast<const uint4*>(src + q * 16 + 8 * half);
+    uint2* d = reinterpret_cast<uint2*>(lds + h * S16 + 8 * half);
+    d[0] = make_uint2(v.x, v.y);
+    d[1] = make_uint2(v.z, v.w);
+  }
+  bf16x4 B[LCP_TAPS];
+#pragma unroll
+  for (int t = 0; t < LCP_TAPS; ++t) B[t] = frags[(F0 + t) * 64 + lane];
+  const float bs = col < CO ? w[lcp_layer_offset(LAYER) + LCP_TAPS * 16 * CO + col] : 0.f;
+  __syncthreads();
+  const int xa = col % 8;
+  float s = 0.f;
+  int c = 0;
+  if (LAYER == 3) {
+    c = clist ? clist[blockIdx.y] : (int)blockIdx.y;
+    s = (float)sqrt(rn2[(size_t)c * rs] / (4.0 * (double)N3));
+  }
+  for (int yp = 0; yp < BY / 2; ++yp) {
+    const int ya = 2 * yp + col / 8, za = wave;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < LCP_TAPS; ++t) {
+      const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+      const int h = ((za + dz) * HY + (ya + dy)) * HX + (xa + dx);
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(lds + h * S16 + 4 * g);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[t], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int row = 4 * g + rr;
+      const int x = Bk.x0 + row % 8, y = Bk.y0 + 2 * yp + row / 8, zz = Bk.z0 + za;
+      if (x >= n || y >= n || zz >= n) continue;
+      const size_t pt = (size_t)x + (size_t)n * ((size_t)y + (size_t)n * zz);
+      if (LAYER < 3) {
+        out[(size_t)blockIdx.y * N3 * 16 + pt * 16 + col] = f2bf(leaky(acc[rr] + bs));
+      } else if (col < 4) {
+        R* __restrict__ zc = reinterpret_cast<R*>(z + (size_t)c * ld);
+        zc[pt * 4 + col] = (R)((acc[rr] + bs) * s * hf[pt]);
+      }
+    }
+  }
+}
+
+inline dim3 pgrid(size_t N3, int nb) { return dim3((unsigned)((N3 + LBS - 1) / LBS), (unsigned)nb); }
+
+// splitmix64 -> uniform (0, 1]
+inline double u01(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return ((z >> 11) + 1) * (1.0 / 9007199254740992.0);
+}
+}  // namespace
+
+int lcp_make_frags(Lcp& L, hipStream_t st);
+
+Lcp::~Lcp() {
+  for (float* p : {w, g, m, v, part, x0, a1, a2, a3, y, t, gy, g1, g2})
+    if (p) (void)hipFree(p);
+  if (loss) (void)hipFree(loss);
+  if (frags) (void)hipFree(frags);
+  for (auto& pr : pairs) (void)hipFree(pr.data);
+}
+
+int Lcp::ensure_scratch(size_t pts) {
+  if (pts <= scratch_pts) return NHF_OK;
+  for (float** p : {&x0, &a1, &a2, &a3, &y, &t, &gy, &g1, &g2})
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  scratch_pts = 0;
+  const size_t sz[9] = {LCP_IN_STRIDE, LCP_HID, LCP_HID, LCP_HID, 4, 4, 4, LCP_HID, LCP_HID};
+  float** ptr[9] = {&x0, &a1, &a2, &a3, &y, &t, &gy, &g1, &g2};
+  for (int i = 0; i < 9; ++i)
+    if (hipMalloc(ptr[i], sizeof(float) * sz[i] * pts) != hipSuccess) return NHF_ENOMEM;
+  scratch_pts = pts;
+  return NHF_OK;
+}
+
+int lcp_init(Lcp& L, unsigned long long seed) {
+  if (!L.w) {
+    for (float** p : {&L.w, &L.g, &L.m, &L.v})
+      if (hipMalloc(p, sizeof(float) * LCP_NPARAM) != hipSuccess) return NHF_ENOMEM;
+    size_t mx = 0;
+    for (int l = 0; l < LCP_LAYERS; ++l) mx = std::max<size_t>(mx, lcp_layer_size(l));
+    if (hipMalloc(&L.part, sizeof(float) * mx * BWD_W_BLOCKS) != hipSuccess) return NHF_ENOMEM;
+    if (hipMalloc(&L.loss, sizeof(double) * 16) != hipSuccess) return NHF_ENOMEM;
+  }
+  // He initialisation: W ~ N(0, 2 / fan_in), fan_in = 27 ci; biases zero
+  std::vector<float> h(LCP_NPARAM, 0.f);
+  uint64_t s = seed;
+  for (int l = 0; l < LCP_LAYERS; ++l) {
+    const int nw = LCP_TAPS * LCP_CI[l] * LCP_CO[l];
+    const double sd = std::sqrt(2.0 / (LCP_TAPS * LCP_CI[l]));
+    float* w = h.data() + lcp_layer_offset(l);
+    for (int i = 0; i < nw; ++i) {
+      const double u1 = u01(s), u2 = u01(s);
+      w[i] = (float)(sd * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * 3.14159265358979323846 * u2));
+    }
+  }
+  if (hipMemcpy(L.w, h.data(), sizeof(float) * LCP_NPARAM, hipMemcpyHostToDevice) != hipSuccess) return NHF_EHIP;
+  if (hipMemset(L.m, 0, sizeof(float) * LCP_NPARAM) != hipSuccess) return NHF_EHIP;
+  if (hipMemset(L.v, 0, sizeof(float) * LCP_NPARAM) != hipSuccess) return NHF_EHIP;
+  L.adam_t = 0;
+  L.ready = true;
+  return lcp_make_frags(L, nullptr);
+}
+
+template <typename R>
+void launch_lcp_planes(hipStream_t st, const R* fld, size_t N3, double h, double shift, float* kap, float* hf) {
+  hipLaunchKernelGGL((k_planes<R>), dim3((unsigned)((N3 + LBS - 1) / LBS)), dim3(LBS), 0, st, fld, N3, h * h, shift,
+                     kap, hf);
+}
+template void launch_lcp_planes<double>(hipStream_t, const double*, size_t, double, double, float*, float*);
+template void launch_lcp_planes<float>(hipStream_t, const float*, size_t, double, double, float*, float*);
+
+template <typename R>
+void lcp_apply(const Lcp& L, hipStream_t st, const Sp<R>* r, Sp<R>* z, size_t ld, int nb, const int* clist,
+               const double* rn2, const float* kap, const float* hf, int n, float* h0, float* h1, int rs) {
+  const dim3 g(bricks_of(n), (unsigned)nb);
+  const bf16x4* fr = reinterpret_cast<const bf16x4*>(L.frags);
+  unsigned short* a0 = reinterpret_cast<unsigned short*>(h0);
+  unsigned short* a1 = reinterpret_cast<unsigned short*>(h1);
+  hipLaunchKernelGGL((k_m_in<R>), g, dim3(LBS), 0, st, r, ld, clist, rn2, kap, fr, L.w, a0, n, rs);
+  hipLaunchKernelGGL((k_m_hid<1, R>), g, dim3(LBS), 0, st, a0, fr, L.w, a1, z, ld, clist, rn2, hf, n, rs);
+  hipLaunchKernelGGL((k_m_hid<2, R>), g, dim3(LBS), 0, st, a1, fr, L.w, a0, z, ld, clist, rn2, hf, n, rs);
+  hipLaunchKernelGGL((k_m_hid<3, R>), g, dim3(LBS), 0, st, a0, fr, L.w, a1, z, ld, clist, rn2, hf, n, rs);
+}
+// FP32 inference (the training kernels), kept for cross-checks of the BF16 path
+template <typename R>
+void lcp_apply_fp32(const Lcp& L, hipStream_t st, const Sp<R>* r, Sp<R>* z, size_t ld, int nb, const int* clist,
+                    const double* rn2, const float* kap, const float* hf, int n, float* h0, float* h1) {
+  const size_t N3 = (size_t)n * n * n;
+  const dim3 g = pgrid(N3, nb);
+  hipLaunchKernelGGL((k_in<R>), g, dim3(LBS), 0, st, r, ld, clist, rn2, kap, L.w + lcp_layer_offset(0), h0, n);
+  hipLaunchKernelGGL((k_fwd<16, 16, 16, true>), g, dim3(LBS), 0, st, h0, N3 * 16, L.w + lcp_layer_offset(1), h1,
+                     N3 * 16, n);
+  hipLaunchKernelGGL((k_fwd<16, 16, 16, true>), g, dim3(LBS), 0, st, h1, N3 * 16, L.w + lcp_layer_offset(2), h0,
+                     N3 * 16, n);
+  hipLaunchKernelGGL((k_out<R>), g, dim3(LBS), 0, st, h0, L.w + lcp_layer_offset(3), z, ld, clist, rn2, hf, n);
+}
+template void lcp_apply_fp32<float>(const Lcp&, hipStream_t, const Sp<float>*, Sp<float>*, size_t, int, const int*,
+                                    const double*, const float*, const float*, int, float*, float*);
+int lcp_make_frags(Lcp& L, hipStream_t st) {
+  if (!L.frags && hipMalloc(&L.frags, sizeof(short) * 4 * 64 * FRAG_N) != hipSuccess) return NHF_ENOMEM;
+  hipLaunchKernelGGL(k_make_frags, dim3((FRAG_N * 64 + LBS - 1) / LBS), dim3(LBS), 0, st, L.w,
+                     reinterpret_cast<bf16x4*>(L.frags));
+  return hipStreamSynchronize(st) == hipSuccess ? NHF_OK : NHF_EHIP;
+}
+template void lcp_apply<double>(const Lcp&, hipStream_t, const Sp<double>*, Sp<double>*, size_t, int, const int*,
+                                const double*, const float*, const float*, int, float*, float*, int);
+template void lcp_apply<float>(const Lcp&, hipStream_t, const Sp<float>*, Sp<float>*, size_t, int, const int*,
+                               const double*, const float*, const float*, int, float*, float*, int);
+
+template <typename R>
+int lcp_add_pairs(Lcp& L, hipStream_t st, const Sp<R>* T, const Sp<R>* X, const Sp<R>* W, const double* ls,
+                  int ncols, size_t ld, const float* kap, const float* hf, int n, double* scratch2) {
+  std::lock_guard<std::mutex> lock(L.mu);
+  const size_t N3 = (size_t)n * n * n;
+  ncols = std::min(ncols, L.collect_left);
+  if (ncols <= 0) return NHF_OK;
+  if (hipMemsetAsync(scratch2, 0, sizeof(double) * 2 * ncols, st) != hipSuccess) return NHF_EHIP;
+  double* rn2 = scratch2;
+  double* tn2 = scratch2 + ncols;
+  const unsigned nbk = (unsigned)std::min<size_t>((N3 + LBS - 1) / LBS, 64);
+  hipLaunchKernelGGL((k_pair_rn2<R>), dim3(nbk, ncols), dim3(LBS), 0, st, T, X, ls, ld, N3, rn2);
+  std::vector<double> hr(ncols);
+  if (hipMemcpyAsync(hr.data(), rn2, sizeof(double) * ncols, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return NHF_EHIP;
+  std::vector<float*> dst(ncols, nullptr);
+  for (int c = 0; c < ncols; ++c) {
+    if (!(hr[c] > 0.0) || !std::isfinite(hr[c])) continue;
+    float* d = nullptr;
+    if (hipMalloc(&d, sizeof(float) * N3 * (LCP_IN_STRIDE + 4)) != hipSuccess) return NHF_ENOMEM;
+    dst[c] = d;
+  }
+  float** ddst = nullptr;
+  if (hipMalloc(&ddst, sizeof(float*) * ncols) != hipSuccess) return NHF_ENOMEM;
+  // columns without a residual get a dummy destination (skipped below)
+  std::vector<float*> dd = dst;
+  float* dummy = nullptr;
+  for (auto& d : dd)
+    if (!d) {
+      if (!dummy && hipMalloc(&dummy, sizeof(float) * N3 * (LCP_IN_STRIDE + 4)) != hipSuccess) return NHF_ENOMEM;
+      d = dummy;
+    }
+  int rc = NHF_OK;
+  if (hipMemcpyAsync(ddst, dd.data(), sizeof(float*) * ncols, hipMemcpyHostToDevice, st) != hipSuccess)
+    rc = NHF_EHIP;
+  if (rc == NHF_OK) {
+    hipLaunchKernelGGL((k_pair_store<R>), dim3(nbk, ncols), dim3(LBS), 0, st, T, X, W, ls, ld, N3, rn2, kap, hf,
+                       ddst);
+    // pair weights |A t|^2
+    for (int c = 0; c < ncols; ++c)
+      hipLaunchKernelGGL(k_aop_norm, pgrid(N3, 1), dim3(LBS), 0, st, nullptr, dd[c] + N3 * LCP_IN_STRIDE, dd[c], N3,
+                         n, LcpWts{}, nullptr, tn2 + c, 1);
+    std::vector<double> ht(ncols);
+    if (hipMemcpyAsync(ht.data(), tn2, sizeof(double) * ncols, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = NHF_EHIP;
+    for (int c = 0; c < ncols && rc == NHF_OK; ++c) {
+      if (!dst[c]) continue;
+      if (!(ht[c] > 0.0) || !std::isfinite(ht[c])) {
+        (void)hipFree(dst[c]);
+        continue;
+      }
+      L.pairs.push_back(LcpPair{n, dst[c], ht[c]});
+      --L.collect_left;
+    }
+  }
+  (void)hipFree(ddst);
+  if (dummy) (void)hipFree(dummy);
+  return rc;
+}
+template int lcp_add_pairs<double>(Lcp&, hipStream_t, const Sp<double>*, const Sp<double>*, const Sp<double>*,
+                                   const double*, int, size_t, const float*, const float*, int, double*);
+template int lcp_add_pairs<float>(Lcp&, hipStream_t, const Sp<float>*, const Sp<float>*, const Sp<float>*,
+                                  const double*, int, size_t, const float*, const float*, int, double*);
+
+void lcp_forward_batch(Lcp& L, hipStream_t st, int n, int nb) {
+  const size_t N3 = (size_t)n * n * n;
+  const dim3 g = pgrid(N3, nb);
+  hipLaunchKernelGGL((k_fwd<5, LCP_IN_STRIDE, 16, true>), g, dim3(LBS), 0, st, L.x0, N3 * LCP_IN_STRIDE,
+                     L.w + lcp_layer_offset(0), L.a1, N3 * 16, n);
+  hipLaunchKernelGGL((k_fwd<16, 16, 16, true>), g, dim3(LBS), 0, st, L.a1, N3 * 16, L.w + lcp_layer_offset(1), L.a2,
+                     N3 * 16, n);
+  hipLaunchKernelGGL((k_fwd<16, 16, 16, true>), g, dim3(LBS), 0, st, L.a2, N3 * 16, L.w + lcp_layer_offset(2), L.a3,
+                     N3 * 16, n);
+  hipLaunchKernelGGL((k_fwd<16, 16, 4, false>), g, dim3(LBS), 0, st, L.a3, N3 * 16, L.w + lcp_layer_offset(3), L.y,
+                     N3 * 4, n);
+}
+
+template <int CIr, int CIs, int CO>
+static void bwd_w(Lcp& L, hipStream_t st, const float* in, size_t in_ss, const float* gz, size_t gz_ss, int n, int nb,
+                  int layer) {
+  const int cnt = lcp_layer_size(layer);
+  hipLaunchKernelGGL((k_bwd_w<CIr, CIs, CO>), dim3(BWD_W_BLOCKS), dim3(LBS), 0, st, in, in_ss, gz, gz_ss, n, nb,
+                     L.part);
+  hipLaunchKernelGGL(k_reduce_part, dim3((cnt + LBS - 1) / LBS), dim3(LBS), 0, st, L.part, BWD_W_BLOCKS, cnt,
+                     L.g + lcp_layer_offset(layer));
+}
+
+void lcp_loss_grad_batch(Lcp& L, hipStream_t st, int n, int nb, const double* wts) {
+  const size_t N3 = (size_t)n * n * n;
+  const dim3 g = pgrid(N3, nb);
+  lcp_forward_batch(L, st, n, nb);
+  LcpWts w{};
+  for (int b = 0; b < nb && b < 16; ++b) w.w[b] = wts[b];
+  hipLaunchKernelGGL(k_aop_norm, g, dim3(LBS), 0, st, L.y, L.t, L.x0, N3, n, w, L.g2, L.loss, 0);
+  hipLaunchKernelGGL(k_aop_grad, g, dim3(LBS), 0, st, L.g2, L.x0, N3, n, w, L.gy);
+  // layer 3 (linear output)
+  bwd_w<16, 16, 4>(L, st, L.a3, N3 * 16, L.gy, N3 * 4, n, nb, 3);
+  hipLaunchKernelGGL((k_bwd_data<16, 16, 4>), g, dim3(LBS), 0, st, L.gy, N3 * 4, L.w + lcp_layer_offset(3), L.a3,
+                     N3 * 16, L.g1, N3 * 16, n);
+  bwd_w<16, 16, 16>(L, st, L.a2, N3 * 16, L.g1, N3 * 16, n, nb, 2);
+  hipLaunchKernelGGL((k_bwd_data<16, 16, 16>), g, dim3(LBS), 0, st, L.g1, N3 * 16, L.w + lcp_layer_offset(2), L.a2,
+                     N3 * 16, L.g2, N3 * 16, n);
+  bwd_w<16, 16, 16>(L, st, L.a1, N3 * 16, L.g2, N3 * 16, n, nb, 1);
+  hipLaunchKernelGGL((k_bwd_data<16, 16, 16>), g, dim3(LBS), 0, st, L.g2, N3 * 16, L.w + lcp_layer_offset(1), L.a1,
+                     N3 * 16, L.g1, N3 * 16, n);
+  bwd_w<5, LCP_IN_STRIDE, 16>(L, st, L.x0, N3 * LCP_IN_STRIDE, L.g1, N3 * 16, n, nb, 0);
+}
+
+void lcp_adam(Lcp& L, hipStream_t st, float lr) {
+  const float b1 = 0.9f, b2 = 0.999f;
+  ++L.adam_t;
+  const float c1 = 1.f / (1.f - std::pow(b1, (float)L.adam_t));
+  const float c2 = 1.f / (1.f - std::pow(b2, (float)L.adam_t));
+  hipLaunchKernelGGL(k_adam, dim3((LCP_NPARAM + LBS - 1) / LBS), dim3(LBS), 0, st, L.w, L.g, L.m, L.v, LCP_NPARAM,
+                     lr, b1, b2, 1e-8f, c1, c2);
+}
+
+// ---- the learned W-block: preconditioned minimal-residual steps on
+// (H + shift) w = ls x, the residual kept as r = (H + shift) w - ls x (the
+// sign the pairs are stored with):
+//   z = M(r),  q = (H + shift) z,  alpha = q^H r / q^H q,  w -= alpha z,
+//   r -= alpha q,
+// so |r| never grows.  Scalars per iteration and column (LW_NS doubles):
+// [0] |r|^2, [1..2] q^H r, [3] q^H q.
+template <typename R>
+__global__ __launch_bounds__(LBS) void k_lw_init(const Sp<R>* __restrict__ X, const Sp<R>* __restrict__ T,
+                                                 const double* __restrict__ ls, Sp<R>* __restrict__ W,
+                                                 Sp<R>* __restrict__ Rr, size_t N3, size_t ld,
+                                                 double* __restrict__ sc) {
+  const int c = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  double v[1] = {0.0};
+  if (p < N3) {
+    const size_t o = (size_t)c * ld + p;
+    const Sp<R> x = X[o], t = T[o];
+    const R l = (R)ls[c];
+    const Sp<R> r{t.ur - l * x.ur, t.ui - l * x.ui, t.dr - l * x.dr, t.di - l * x.di};
+    W[o] = x;
+    Rr[o] = r;
+    v[0] = (double)r.ur * r.ur + (double)r.ui * r.ui + (double)r.dr * r.dr + (double)r.di * r.di;
+  }
+  block_sum_add<1>(v, sc + (size_t)c * LW_NS);
+}
+template <typename R>
+__global__ __launch_bounds__(LBS) void k_lw_dots(const Sp<R>* __restrict__ Q, const Sp<R>* __restrict__ Rr, size_t N3,
+                                                 size_t ld, double* __restrict__ sc) {
+  const int c = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (p < N3) {
+    const size_t o = (size_t)c * ld + p;
+    const Sp<R> q = Q[o], r = Rr[o];
+    v[0] = (double)q.ur * r.ur + (double)q.ui * r.ui + (double)q.dr * r.dr + (double)q.di * r.di;
+    v[1] = (double)q.ur * r.ui - (double)q.ui * r.ur + (double)q.dr * r.di - (double)q.di * r.dr;
+    v[2] = (double)q.ur * q.ur + (double)q.ui * q.ui + (double)q.dr * q.dr + (double)q.di * q.di;
+  }
+  block_sum_add<3>(v, sc + (size_t)c * LW_NS + 1);
+}
+template <typename R>
+__global__ __launch_bounds__(LBS) void k_lw_update(const Sp<R>* __restrict__ Z, const Sp<R>* __restrict__ Q,
+                                                   Sp<R>* __restrict__ W, Sp<R>* __restrict__ Rr, size_t N3,
+                                                   size_t ld, const double* __restrict__ sc,
+                                                   double* __restrict__ sc_next) {
+  const int c = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  const double* s = sc + (size_t)c * LW_NS;
+  const double qq = s[3];
+  const double ar = qq > 0.0 ? s[1] / qq : 0.0, ai = qq > 0.0 ? s[2] / qq : 0.0;
+  double v[1] = {0.0};
+  if (p < N3) {
+    const size_t o = (size_t)c * ld + p;
+    const Sp<R> z = Z[o], q = Q[o];
+    Sp<R> w = W[o], r = Rr[o];
+    const R a = (R)ar, b = (R)ai;
+    w.ur -= a * z.ur - b * z.ui;
+    w.ui -= a * z.ui + b * z.ur;
+    w.dr -= a * z.dr - b * z.di;
+    w.di -= a * z.di + b * z.dr;
+    r.ur -= a * q.ur - b * q.ui;
+    r.ui -= a * q.ui + b * q.ur;
+    r.dr -= a * q.dr - b * q.di;
+    r.di -= a * q.di + b * q.dr;
+    W[o] = w;
+    Rr[o] = r;
+    v[0] = (double)r.ur * r.ur + (double)r.ui * r.ui + (double)r.dr * r.dr + (double)r.di * r.di;
+  }
+  block_sum_add<1>(v, sc_next + (size_t)c * LW_NS);
+}
+// random global phase of a training sample (input channels 0-3 and target)
+__global__ __launch_bounds__(LBS) void k_phase(float* __restrict__ in8, float* __restrict__ tg, size_t N3, float cs,
+                                               float sn) {
+  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  if (p >= N3) return;
+  float* a = in8 + p * LCP_IN_STRIDE;
+  float* b = tg + p * 4;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const float ar = a[2 * s], ai = a[2 * s + 1], br = b[2 * s], bi = b[2 * s + 1];
+    a[2 * s] = cs * ar - sn * ai;
+    a[2 * s + 1] = sn * ar + cs * ai;
+    b[2 * s] = cs * br - sn * bi;
+    b[2 * s + 1] = sn * br + cs * bi;
+  }
+}
+
+template <typename R>
+void launch_lw_init(hipStream_t st, const Sp<R>* X, const Sp<R>* T, const double* ls, Sp<R>* W, Sp<R>* Rr, size_t N3,
+                    size_t ld, int m, double* sc) {
+  hipLaunchKernelGGL((k_lw_init<R>), pgrid(N3, m), dim3(LBS), 0, st, X, T, ls, W, Rr, N3, ld, sc);
+}
+template <typename R>
+void launch_lw_dots(hipStream_t st, const Sp<R>* Q, const Sp<R>* Rr, size_t N3, size_t ld, int m, double* sc) {
+  hipLaunchKernelGGL((k_lw_dots<R>), pgrid(N3, m), dim3(LBS), 0, st, Q, Rr, N3, ld, sc);
+}
+template <typename R>
+void launch_lw_update(hipStream_t st, const Sp<R>* Z, const Sp<R>* Q, Sp<R>* W, Sp<R>* Rr, size_t N3, size_t ld, int m,
+                      const double* sc, double* sc_next) {
+  hipLaunchKernelGGL((k_lw_update<R>), pgrid(N3, m), dim3(LBS), 0, st, Z, Q, W, Rr, N3, ld, sc, sc_next);
+}
+#define LW_INST(R)                                                                                              \
+  template void launch_lw_init<R>(hipStream_t, const Sp<R>*, const Sp<R>*, const double*, Sp<R>*, Sp<R>*, size_t, \
+                                  size_t, int, double*);                                                        \
+  template void launch_lw_dots<R>(hipStream_t, const Sp<R>*, const Sp<R>*, size_t, size_t, int, double*);        \
+  template void launch_lw_update<R>(hipStream_t, const Sp<R>*, const Sp<R>*, Sp<R>*, Sp<R>*, size_t, size_t, int, \
+                                    const double*, double*);
+LW_INST(double)
+LW_INST(float)
+#undef LW_INST
+
+int lcp_target_norms(Lcp& L, hipStream_t st, int n, int nb, double* out) {
+  const size_t N3 = (size_t)n * n * n;
+  if (hipMemsetAsync(L.loss, 0, sizeof(double) * 16, st) != hipSuccess) return NHF_EHIP;
+  hipLaunchKernelGGL(k_aop_norm, pgrid(N3, nb), dim3(LBS), 0, st, nullptr, L.t, L.x0, N3, n, LcpWts{}, nullptr,
+                     L.loss, 1);
+  if (hipMemcpyAsync(out, L.loss, sizeof(double) * nb, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return NHF_EHIP;
+  return NHF_OK;
+}
+
+int lcp_train(Lcp& L, hipStream_t st, int steps, int batch, double lr, unsigned long long seed, double* loss_first,
+              double* loss_last) {
+  if (!L.ready || L.pairs.empty() || steps < 1 || batch < 1 || batch > 16) return NHF_EINVAL;
+  // pairs grouped by lattice size; a step draws its batch from one group,
+  // chosen with probability proportional to the group's size
+  std::vector<int> sizes;
+  for (const auto& p : L.pairs)
+    if (std::find(sizes.begin(), sizes.end(), p.n) == sizes.end()) sizes.push_back(p.n);
+  std::vector<std::vector<int>> grp(sizes.size());
+  for (int i = 0; i < (int)L.pairs.size(); ++i)
+    grp[std::find(sizes.begin(), sizes.end(), L.pairs[i].n) - sizes.begin()].push_back(i);
+  size_t mx = 0;
+  for (int nn : sizes) mx = std::max(mx, (size_t)nn * nn * nn);
+  int rc = L.ensure_scratch(mx * batch);
+  if (rc != NHF_OK) return rc;
+  double* dl = nullptr;
+  if (hipMalloc(&dl, sizeof(double) * steps) != hipSuccess) return NHF_ENOMEM;
+  if (hipMemsetAsync(dl, 0, sizeof(double) * steps, st) != hipSuccess) rc = NHF_EHIP;
+  uint64_t s = seed;
+  for (int it = 0; it < steps && rc == NHF_OK; ++it) {
+    double u = u01(s) * (double)L.pairs.size();
+    int gi = 0;
+    while (gi + 1 < (int)grp.size() && u > (double)grp[gi].size()) u -= (double)grp[gi++].size();
+    const int n = sizes[gi];
+    const size_t N3 = (size_t)n * n * n;
+    double wts[16];
+    for (int b = 0; b < batch; ++b) {
+      const int k = grp[gi][std::min((size_t)(u01(s) * grp[gi].size()), grp[gi].size() - 1)];
+      const LcpPair& P = L.pairs[k];
+      if (hipMemcpyAsync(L.x0 + (size_t)b * N3 * LCP_IN_STRIDE, P.data, sizeof(float) * N3 * LCP_IN_STRIDE,
+                         hipMemcpyDeviceToDevice, st) != hipSuccess ||
+          hipMemcpyAsync(L.t + (size_t)b * N3 * 4, P.data + N3 * LCP_IN_STRIDE, sizeof(float) * N3 * 4,
+                         hipMemcpyDeviceToDevice, st) != hipSuccess)
+        rc = NHF_EHIP;
+      wts[b] = 1.0 / (batch * P.tn2);
+      // the operator is complex linear: a random global phase per sample
+      const double ph = 2.0 * 3.14159265358979323846 * u01(s);
+      hipLaunchKernelGGL(k_phase, dim3((unsigned)((N3 + LBS - 1) / LBS)), dim3(LBS), 0, st,
+                         L.x0 + (size_t)b * N3 * LCP_IN_STRIDE, L.t + (size_t)b * N3 * 4, N3, (float)std::cos(ph),
+                         (float)std::sin(ph));
+    }
+    double* keep = L.loss;
+    L.loss = dl + it;
+    lcp_loss_grad_batch(L, st, n, batch, wts);
+    L.loss = keep;
+    lcp_adam(L, st, (float)lr);
+  }
+  std::vector<double> h(steps);
+  if (rc == NHF_OK && (hipMemcpyAsync(h.data(), dl, sizeof(double) * steps, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                       hipStreamSynchronize(st) != hipSuccess))
+    rc = NHF_EHIP;
+  (void)hipFree(dl);
+  if (rc != NHF_OK) return rc;
+  const int w = std::max(1, std::min(10, steps / 2));
+  double a = 0, b = 0;
+  for (int i = 0; i < w; ++i) {
+    a += h[i];
+    b += h[steps - 1 - i];
+  }
+  if (loss_first) *loss_first = a / w;
+  if (loss_last) *loss_last = b / w;
+  if (!std::isfinite(b)) return NHF_ESOLVER;
+  return lcp_make_frags(L, st);
+}
+
+}  // namespace nhf

@@ -14,6 +14,7 @@
 #include <tuple>
 
 #include "nhf_internal.h"
+#include "nhf_lcp.h"
 #include "nhf_pairing.h"
 
 namespace nhf {
@@ -22,6 +23,7 @@ constexpr double HBAR = 197.327053;
 constexpr double MNUC = 938.91869;
 constexpr double PI_ = 3.14159265358979323846;
 constexpr size_t DENS_PLANES = 24;
+constexpr int LW_MAX_ITERS = 8;  // learned W-block iterations per call
 
 static thread_local hipError_t g_last_hip = hipSuccess;
 
@@ -67,6 +69,12 @@ struct Work {
   unsigned long long* act = nullptr;  // active column-launches (device counter)
   // wall-time breakdown of the species step (seconds, accumulated)
   double tph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // learned W-block: per-point scales, net scratch, step scalars, statistics
+  float *kap = nullptr, *hf = nullptr, *lh0 = nullptr, *lh1 = nullptr;
+  double *lw = nullptr, *pscr = nullptr;
+  int lcp_cool = 0;
+  long gcg_calls = 0, lcp_calls = 0, lcp_fallbacks = 0;
+  double lcp_ratio_sum = 0.0;
 };
 // phases: 0 fields (+ Coulomb), 1 guess orthonormalisation + first RR,
 // 2 block CG, 3 basis + Cholesky-QR, 4 projection + RR eigensolve + rotation,
@@ -124,6 +132,8 @@ struct Problem {
   std::vector<ConState> cons;
   double energyDiff = 1.0;
   double massCorr = MNUC;
+  Lcp* lcp = nullptr;       // the context's learned preconditioner
+  double lcp_stall = 0.9;   // mean |r_k| / |r_0| above which the W-block falls back
   ~Problem() {
     for (int q = 0; q < 2; ++q) {
       for (hipEvent_t e : W[q].ev) (void)hipEventDestroy(e);
@@ -148,6 +158,7 @@ struct Ctx {
   int device = 0;
   hipStream_t st = nullptr;
   std::vector<std::unique_ptr<Problem>> probs;
+  Lcp lcp;
 };
 
 static Problem* get_problem(void* ctx, int pid) {
@@ -248,6 +259,14 @@ static int setup_problem(Problem& P) {
     CHK(P.alloc(W.its, P.Mmax));
     CHK(P.alloc(W.clist, P.Mmax));
     CHK(P.alloc(W.act, 1));
+    CHK(P.alloc(W.kap, N3));
+    CHK(P.alloc(W.hf, N3));
+    CHK(P.alloc(W.pscr, 2 * (size_t)P.Mmax));
+    CHK(P.alloc(W.lw, (size_t)LW_NS * (LW_MAX_ITERS + 1) * P.Mmax));
+    if (c.wstep == NHF_W_LEARNED) {
+      CHK(P.alloc(W.lh0, (size_t)P.Mmax * N3 * LCP_HID));
+      CHK(P.alloc(W.lh1, (size_t)P.Mmax * N3 * LCP_HID));
+    }
   }
   P.pmaxit = 20000;
   CHK(P.alloc(P.prhs, N3));
@@ -332,6 +351,47 @@ const float* fld_of<float>(Problem& P, int q) {
   return P.fld32[q];
 }
 
+// The learned W-block: `iters` preconditioned minimal-residual steps on
+// (H + shift) w = (lambda + shift) x from w = x, with the context's
+// convolutional net as the preconditioner.  `ratio` = mean over columns of
+// |r_k| / |r_0| (columns with r_0 != 0).  Buffers: V1[0:M] residual,
+// V1[M:2M] net output, V1[2M:3M] (H + shift) z.
+template <typename R>
+static int learned_wstep(Problem& P, Work& Wk, int q, const Sp<R>* X, Sp<R>* Wb, double shift, int M, int iters,
+                         double& ratio) {
+  hipStream_t st = Wk.st;
+  const size_t N3 = P.N3, ld = N3;
+  const R* fld = fld_of<R>(P, q);
+  Sp<R>* Rr = (Sp<R>*)Wk.V1;
+  Sp<R>* Z = Rr + (size_t)M * ld;
+  Sp<R>* Q = Rr + (size_t)2 * M * ld;
+  auto sc = [&](int it) { return Wk.lw + (size_t)it * M * LW_NS; };
+  launch_lcp_planes<R>(st, fld, N3, P.h, shift, Wk.kap, Wk.hf);
+  HIPCHK(hipMemsetAsync(Wk.lw, 0, sizeof(double) * LW_NS * (LW_MAX_ITERS + 1) * M, st));
+  launch_hpsi<R>(st, X, Q, ld, M, fld, P.n, P.h, shift, nullptr, 0, nullptr);
+  launch_lw_init<R>(st, X, Q, Wk.lam_shift, Wb, Rr, N3, ld, M, sc(0));
+  for (int it = 0; it < iters; ++it) {
+    lcp_apply<R>(*P.lcp, st, Rr, Z, ld, M, nullptr, sc(it), Wk.kap, Wk.hf, P.n, Wk.lh0, Wk.lh1, LW_NS);
+    launch_hpsi<R>(st, Z, Q, ld, M, fld, P.n, P.h, shift, nullptr, 0, nullptr);
+    launch_lw_dots<R>(st, Q, Rr, N3, ld, M, sc(it));
+    launch_lw_update<R>(st, Z, Q, Wb, Rr, N3, ld, M, sc(it), sc(it + 1));
+  }
+  std::vector<double> h((size_t)(iters + 1) * M * LW_NS);
+  HIPCHK(hipMemcpyAsync(h.data(), Wk.lw, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double sum = 0.0;
+  int cnt = 0;
+  for (int c = 0; c < M; ++c) {
+    const double r0 = h[(size_t)c * LW_NS], rk = h[((size_t)iters * M + c) * LW_NS];
+    if (!(r0 > 0.0)) continue;
+    sum += std::sqrt(rk / r0);
+    ++cnt;
+  }
+  ratio = cnt ? sum / cnt : 0.0;
+  if (!std::isfinite(ratio)) ratio = 1e30;
+  return NHF_OK;
+}
+
 // ---------------------------------------------------------------- GCG
 // One call of gcgm_complex_no_B_lock (solver.cpp:642-959) with max_iter
 // passes: orthonormalise the guess (Cholesky-QR; same Q as the modified
@@ -378,7 +438,32 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
     Sp<R>* Rr = V1;
     Sp<R>* Pd = V1 + (size_t)M * ld;
     Sp<R>* T = V1 + (size_t)2 * M * ld;
-    // 3. block Jacobi-PCG
+    // 3. W-block: the learned preconditioner when enabled (falls back to the
+    //    plain Jacobi-PCG below when the residual reduction stalls, and skips
+    //    the next two calls of this species), else block Jacobi-PCG
+    ++Wk.gcg_calls;
+    bool w_done = false;
+    if (P.cfg.wstep == NHF_W_LEARNED && P.lcp && P.lcp->ready) {
+      if (Wk.lcp_cool > 0) {
+        --Wk.lcp_cool;
+      } else {
+        std::vector<double> ls(M);
+        for (int c = 0; c < M; ++c) ls[c] = Lam[c] + shift;
+        HIPCHK(hipMemcpyAsync(Wk.lam_shift, ls.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
+        const int iters = std::min(LW_MAX_ITERS, std::max(1, P.cfg.wstep_iters));
+        double ratio = 0.0;
+        CHK(learned_wstep<R>(P, Wk, q, V, Wb, shift, M, iters, ratio));
+        ++Wk.lcp_calls;
+        Wk.lcp_ratio_sum += ratio;
+        if (ratio <= P.lcp_stall) {
+          w_done = true;
+        } else {
+          ++Wk.lcp_fallbacks;
+          Wk.lcp_cool = 2;
+        }
+      }
+    }
+    if (!w_done) {
     launch_invdiag<R>(st, fld, N3, P.h, shift, invd);
     launch_hpsi<R>(st, X, T, ld, M, fld, P.n, P.h, shift, nullptr, 0, nullptr);
     std::vector<double> ls(M);
@@ -447,6 +532,13 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
       }
       Wk.st_launches += nev;
     }
+    // training pairs (residual, converged correction) for the learned net
+    if (P.lcp && P.lcp->collect_left > 0 && (Wk.gcg_calls - 1) % std::max(1, P.lcp->collect_every) == 0) {
+      launch_lcp_planes<R>(st, fld, N3, P.h, shift, Wk.kap, Wk.hf);
+      launch_hpsi<R>(st, X, T, ld, M, fld, P.n, P.h, shift, nullptr, 0, nullptr);
+      CHK(lcp_add_pairs<R>(*P.lcp, st, T, X, Wb, Wk.lam_shift, M, ld, Wk.kap, Wk.hf, P.n, Wk.pscr));
+    }
+    }  // plain W-block
     HIPCHK(hipStreamSynchronize(st));
     auto tC = Clock::now();
     Wk.tph[2] += secs(tB, tC);
@@ -1209,13 +1301,14 @@ int nhf_problem(void* ctx, const nhf_config* cfg, int* pid) {
   }
   if ((cfg->mixing != NHF_MIX_LINEAR && cfg->mixing != NHF_MIX_BROYDEN) || cfg->mix_hist < 0 ||
       cfg->mix_hist > NHF_MAX_HIST || cfg->mix_alpha < 0 || (cfg->wstep != NHF_W_JACOBI_CG && cfg->wstep != NHF_W_LEARNED) ||
-      cfg->wstep_iters < 0)
+      cfg->wstep_iters < 0 || cfg->wstep_iters > LW_MAX_ITERS)
     return NHF_EINVAL;
   Ctx* c = (Ctx*)ctx;
   HIPCHK(hipSetDevice(c->device));
   auto P = std::make_unique<Problem>();
   P->cfg = *cfg;
   P->st = c->st;
+  P->lcp = &c->lcp;
   int rc = setup_problem(*P);
   if (rc != NHF_OK) return rc;
   c->probs.push_back(std::move(P));
@@ -1462,6 +1555,166 @@ int nhf_phase_times(void* ctx, int pid, int reset, double* out) {
     P->tcommon = P->tenergy = 0.0;
     for (int q = 0; q < 2; ++q)
       for (double& t : P->W[q].tph) t = 0.0;
+  }
+  return NHF_OK;
+}
+
+// ------------------------------------------------ learned preconditioner
+int nhf_lcp_num_params(void) { return LCP_NPARAM; }
+
+int nhf_lcp_init(void* ctx, unsigned long long seed) {
+  if (!ctx) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  HIPCHK(hipSetDevice(c->device));
+  return lcp_init(c->lcp, seed);
+}
+
+int nhf_lcp_set_params(void* ctx, const float* w, int count) {
+  if (!ctx || !w || count != LCP_NPARAM) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  HIPCHK(hipSetDevice(c->device));
+  if (!c->lcp.w) CHK(lcp_init(c->lcp, 0));
+  HIPCHK(hipMemcpy(c->lcp.w, w, sizeof(float) * LCP_NPARAM, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->lcp.m, 0, sizeof(float) * LCP_NPARAM));
+  HIPCHK(hipMemset(c->lcp.v, 0, sizeof(float) * LCP_NPARAM));
+  c->lcp.adam_t = 0;
+  c->lcp.ready = true;
+  return lcp_make_frags(c->lcp, c->st);
+}
+
+int nhf_lcp_get_params(void* ctx, float* w, int count) {
+  if (!ctx || !w || count != LCP_NPARAM) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  if (!c->lcp.ready) return NHF_ESTATE;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy(w, c->lcp.w, sizeof(float) * LCP_NPARAM, hipMemcpyDeviceToHost));
+  return NHF_OK;
+}
+
+int nhf_lcp_forward(void* ctx, int n, int batch, const float* in8, float* out4) {
+  if (!ctx || !in8 || !out4 || n < 3 || batch < 1 || batch > 16) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  if (!c->lcp.ready) return NHF_ESTATE;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t N3 = (size_t)n * n * n;
+  CHK(c->lcp.ensure_scratch(N3 * batch));
+  HIPCHK(hipMemcpyAsync(c->lcp.x0, in8, sizeof(float) * N3 * batch * LCP_IN_STRIDE, hipMemcpyHostToDevice, c->st));
+  lcp_forward_batch(c->lcp, c->st, n, batch);
+  HIPCHK(hipMemcpyAsync(out4, c->lcp.y, sizeof(float) * N3 * batch * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return NHF_OK;
+}
+
+int nhf_lcp_loss_grad(void* ctx, int n, int batch, const float* in8, const float* tgt4, double* loss, float* grad) {
+  if (!ctx || !in8 || !tgt4 || !loss || !grad || n < 3 || batch < 1 || batch > 16) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  if (!c->lcp.ready) return NHF_ESTATE;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t N3 = (size_t)n * n * n;
+  CHK(c->lcp.ensure_scratch(N3 * batch));
+  HIPCHK(hipMemcpyAsync(c->lcp.x0, in8, sizeof(float) * N3 * batch * LCP_IN_STRIDE, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->lcp.t, tgt4, sizeof(float) * N3 * batch * 4, hipMemcpyHostToDevice, c->st));
+  double wts[16];
+  CHK(lcp_target_norms(c->lcp, c->st, n, batch, wts));
+  for (int b = 0; b < batch; ++b) {
+    if (!(wts[b] > 0.0)) return NHF_EINVAL;
+    wts[b] = 1.0 / (batch * wts[b]);
+  }
+  HIPCHK(hipMemsetAsync(c->lcp.loss, 0, sizeof(double), c->st));
+  lcp_loss_grad_batch(c->lcp, c->st, n, batch, wts);
+  HIPCHK(hipMemcpyAsync(loss, c->lcp.loss, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(grad, c->lcp.g, sizeof(float) * LCP_NPARAM, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return NHF_OK;
+}
+
+int nhf_lcp_apply(void* ctx, int n, int m, const float* r, const float* kap, const float* hf, float* z, int mode) {
+  if (!ctx || !r || !kap || !hf || !z || n < 3 || m < 1 || (mode != 0 && mode != 1)) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  if (!c->lcp.ready) return NHF_ESTATE;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t N3 = (size_t)n * n * n;
+  std::vector<double> rn2(m, 0.0);
+  for (int j = 0; j < m; ++j)
+    for (size_t i = 0; i < 4 * N3; ++i) rn2[j] += (double)r[j * 4 * N3 + i] * r[j * 4 * N3 + i];
+  void *dr = nullptr, *dz = nullptr, *dk = nullptr, *dh = nullptr, *dn = nullptr, *h0 = nullptr, *h1 = nullptr;
+  int rc = NHF_OK;
+  if (hipMalloc(&dr, sizeof(float) * 4 * N3 * m) != hipSuccess || hipMalloc(&dz, sizeof(float) * 4 * N3 * m) != hipSuccess ||
+      hipMalloc(&dk, sizeof(float) * N3) != hipSuccess || hipMalloc(&dh, sizeof(float) * N3) != hipSuccess ||
+      hipMalloc(&dn, sizeof(double) * m) != hipSuccess || hipMalloc(&h0, sizeof(float) * LCP_HID * N3 * m) != hipSuccess ||
+      hipMalloc(&h1, sizeof(float) * LCP_HID * N3 * m) != hipSuccess)
+    rc = NHF_ENOMEM;
+  if (rc == NHF_OK &&
+      (hipMemcpy(dr, r, sizeof(float) * 4 * N3 * m, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(dk, kap, sizeof(float) * N3, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(dh, hf, sizeof(float) * N3, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(dn, rn2.data(), sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess))
+    rc = NHF_EHIP;
+  if (rc == NHF_OK) {
+    if (mode == 0)
+      lcp_apply<float>(c->lcp, c->st, (const Sp<float>*)dr, (Sp<float>*)dz, N3, m, nullptr, (const double*)dn,
+                       (const float*)dk, (const float*)dh, n, (float*)h0, (float*)h1);
+    else
+      lcp_apply_fp32<float>(c->lcp, c->st, (const Sp<float>*)dr, (Sp<float>*)dz, N3, m, nullptr, (const double*)dn,
+                            (const float*)dk, (const float*)dh, n, (float*)h0, (float*)h1);
+    if (hipStreamSynchronize(c->st) != hipSuccess ||
+        hipMemcpy(z, dz, sizeof(float) * 4 * N3 * m, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = NHF_EHIP;
+  }
+  for (void* p : {dr, dz, dk, dh, dn, h0, h1})
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
+int nhf_lcp_collect(void* ctx, int max_pairs, int every) {
+  if (!ctx || max_pairs < 0 || every < 1) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  std::lock_guard<std::mutex> lock(c->lcp.mu);
+  c->lcp.collect_left = max_pairs;
+  c->lcp.collect_every = every;
+  return NHF_OK;
+}
+
+int nhf_lcp_num_pairs(void* ctx) {
+  if (!ctx) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  std::lock_guard<std::mutex> lock(c->lcp.mu);
+  return (int)c->lcp.pairs.size();
+}
+
+int nhf_lcp_clear_pairs(void* ctx) {
+  if (!ctx) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  std::lock_guard<std::mutex> lock(c->lcp.mu);
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipDeviceSynchronize());
+  for (auto& pr : c->lcp.pairs) (void)hipFree(pr.data);
+  c->lcp.pairs.clear();
+  return NHF_OK;
+}
+
+int nhf_lcp_train(void* ctx, int steps, int batch, double lr, unsigned long long seed, double* loss_first,
+                  double* loss_last) {
+  if (!ctx || lr <= 0) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  HIPCHK(hipSetDevice(c->device));
+  return lcp_train(c->lcp, c->st, steps, batch, lr, seed, loss_first, loss_last);
+}
+
+int nhf_lcp_stats(void* ctx, int pid, int reset, double* out) {
+  Problem* P = get_problem(ctx, pid);
+  if (!P || !out) return NHF_EINVAL;
+  for (int q = 0; q < 2; ++q) {
+    Work& W = P->W[q];
+    out[4 * q + 0] = (double)W.gcg_calls;
+    out[4 * q + 1] = (double)W.lcp_calls;
+    out[4 * q + 2] = (double)W.lcp_fallbacks;
+    out[4 * q + 3] = W.lcp_calls ? W.lcp_ratio_sum / W.lcp_calls : 0.0;
+    if (reset) {
+      W.gcg_calls = W.lcp_calls = W.lcp_fallbacks = 0;
+      W.lcp_ratio_sum = 0.0;
+      W.lcp_cool = 0;
+    }
   }
   return NHF_OK;
 }

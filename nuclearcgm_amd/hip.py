@@ -51,6 +51,20 @@ def lib():
         L.nhf_cholqr_coeffs.argtypes = [ctypes.c_int, _D, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _D]
         L.nhf_edf_from_skyrme.argtypes = [_D, _D]
         L.nhf_edf_from_skyrme.restype = None
+        _F = ctypes.POINTER(ctypes.c_float)
+        L.nhf_lcp_num_params.argtypes = []
+        L.nhf_lcp_init.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong]
+        L.nhf_lcp_set_params.argtypes = [ctypes.c_void_p, _F, ctypes.c_int]
+        L.nhf_lcp_get_params.argtypes = [ctypes.c_void_p, _F, ctypes.c_int]
+        L.nhf_lcp_forward.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _F, _F]
+        L.nhf_lcp_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _F, _F, _D, _F]
+        L.nhf_lcp_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _F, _F, _F, _F, ctypes.c_int]
+        L.nhf_lcp_collect.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.nhf_lcp_num_pairs.argtypes = [ctypes.c_void_p]
+        L.nhf_lcp_clear_pairs.argtypes = [ctypes.c_void_p]
+        L.nhf_lcp_train.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong,
+                                    _D, _D]
+        L.nhf_lcp_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         _lib = L
     return _lib
 
@@ -62,6 +76,14 @@ def _chk(rc):
 
 def _dp(a):
     return a.ctypes.data_as(_D)
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def lcp_num_params():
+    return lib().nhf_lcp_num_params()
 
 
 def _cols(a):
@@ -146,6 +168,60 @@ class Context:
         _chk(lib().nhf_problem(self.h, ctypes.byref(cfg), ctypes.byref(pid)))
         return Problem(self, pid.value, cfg)
 
+    # ---- learned convolutional preconditioner (one per context)
+    def lcp_init(self, seed):
+        _chk(lib().nhf_lcp_init(self.h, seed))
+
+    def lcp_params(self):
+        w = np.zeros(lcp_num_params(), dtype=np.float32)
+        _chk(lib().nhf_lcp_get_params(self.h, _fp(w), w.size))
+        return w
+
+    def lcp_set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _chk(lib().nhf_lcp_set_params(self.h, _fp(w), w.size))
+
+    def lcp_forward(self, x8, n):
+        """x8: (batch, n^3, 8) float32 -> (batch, n^3, 4)."""
+        x8 = np.ascontiguousarray(x8, dtype=np.float32)
+        out = np.zeros((x8.shape[0], n ** 3, 4), dtype=np.float32)
+        _chk(lib().nhf_lcp_forward(self.h, n, x8.shape[0], _fp(x8), _fp(out)))
+        return out
+
+    def lcp_loss_grad(self, x8, t4, n):
+        x8 = np.ascontiguousarray(x8, dtype=np.float32)
+        t4 = np.ascontiguousarray(t4, dtype=np.float32)
+        loss = ctypes.c_double()
+        g = np.zeros(lcp_num_params(), dtype=np.float32)
+        _chk(lib().nhf_lcp_loss_grad(self.h, n, x8.shape[0], _fp(x8), _fp(t4), ctypes.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def lcp_apply(self, r, kap, hf, n, fp32=False):
+        """r: (m, n^3, 4) float32 residual columns -> corrections (m, n^3, 4)."""
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        kap = np.ascontiguousarray(kap, dtype=np.float32)
+        hf = np.ascontiguousarray(hf, dtype=np.float32)
+        z = np.zeros_like(r)
+        _chk(lib().nhf_lcp_apply(self.h, n, r.shape[0], _fp(r), _fp(kap), _fp(hf), _fp(z), 1 if fp32 else 0))
+        return z
+
+    def lcp_collect(self, max_pairs, every=1):
+        _chk(lib().nhf_lcp_collect(self.h, max_pairs, every))
+
+    def lcp_num_pairs(self):
+        r = lib().nhf_lcp_num_pairs(self.h)
+        if r < 0:
+            _chk(r)
+        return r
+
+    def lcp_clear_pairs(self):
+        _chk(lib().nhf_lcp_clear_pairs(self.h))
+
+    def lcp_train(self, steps, batch=4, lr=1e-3, seed=1):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _chk(lib().nhf_lcp_train(self.h, steps, batch, lr, seed, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
 
 class Problem:
     def __init__(self, ctx, pid, cfg):
@@ -215,6 +291,15 @@ class Problem:
             for k, name in enumerate(self.PHASES):
                 d[f"{sp}:{name}"] = out[2 + 6 * q + k]
         return d
+
+    def lcp_stats(self, reset=False):
+        """Learned W-block statistics per species: GCG calls, learned steps,
+        fallbacks to Jacobi-PCG, mean residual reduction |r_k|/|r_0|."""
+        out = np.zeros(8)
+        _chk(lib().nhf_lcp_stats(self.ctx.h, self.pid, int(reset), _dp(out)))
+        return {sp: {"gcg_calls": int(out[4 * q]), "learned": int(out[4 * q + 1]),
+                     "fallbacks": int(out[4 * q + 2]), "mean_ratio": float(out[4 * q + 3])}
+                for q, sp in enumerate("np")}
 
     def scf_run(self, psi_n, psi_p, eps_n, eps_p, trace_cap=4000):
         psi_n = _cols(psi_n).copy(order="F")

@@ -1812,4 +1812,160 @@ int orc_fields_once(const nhf_config* cfg, const double* psi_n, const double* ps
   return 0;
 }
 
+// ---------------------------------------------------------------------
+// Learned convolutional preconditioner (no reference counterpart): plain
+// FP64 restatement of the net used by the library -- four 3x3x3
+// convolutions 5 -> 16 -> 16 -> 16 -> 4 channels with zero padding,
+// LeakyReLU(0.1) after the first three; parameters per layer W[tap][ci][co]
+// then b[co], tap = (dz+1)*9 + (dy+1)*3 + (dx+1); input stored with 8
+// channels per point (5 used), output 4 channels.  Loss
+// sum_b |A (y_b - t_b)|^2 / (B |A t_b|^2), A = -lap4 + kappa in lattice
+// units (13-point 4th-order Laplacian, zero outside the box, kappa = input
+// channel 4), and its gradient by backpropagation.
+namespace lcp {
+const int L = 4, CI[4] = {5, 16, 16, 16}, CO[4] = {16, 16, 16, 4};
+const double SLOPE = 0.1;
+inline int lsize(int l) { return 27 * CI[l] * CO[l] + CO[l]; }
+inline int loff(int l) { return l == 0 ? 0 : loff(l - 1) + lsize(l - 1); }
+
+// out[p][co] = b[co] + sum_tap sum_ci in[p + off(tap)][ci] W[tap][ci][co]
+static void conv(int n, const double* in, int cis, int ci_n, const double* w, int co_n, double* out) {
+  const size_t N3 = (size_t)n * n * n;
+  const double* b = w + 27 * ci_n * co_n;
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const size_t p = i + (size_t)n * (j + (size_t)n * k);
+        double* o = out + p * co_n;
+        for (int co = 0; co < co_n; ++co) o[co] = b[co];
+        for (int dz = -1; dz <= 1; ++dz)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+              const int ii = i + dx, jj = j + dy, kk = k + dz;
+              if (ii < 0 || jj < 0 || kk < 0 || ii >= n || jj >= n || kk >= n) continue;
+              const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+              const double* x = in + (ii + (size_t)n * (jj + (size_t)n * kk)) * cis;
+              for (int ci = 0; ci < ci_n; ++ci)
+                for (int co = 0; co < co_n; ++co) o[co] += x[ci] * w[(tap * ci_n + ci) * co_n + co];
+            }
+      }
+  (void)N3;
+}
+
+// out = A v on a 4-channel field (A symmetric)
+static void aop(int n, const double* v, const float* in8, double* out) {
+  const size_t N3 = (size_t)n * n * n;
+  const size_t st[3] = {1, (size_t)n, (size_t)n * n};
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const size_t p = i + (size_t)n * (j + (size_t)n * k);
+        const int pos[3] = {i, j, k};
+        const double kap = in8[p * 8 + 4];
+        for (int c = 0; c < 4; ++c) out[p * 4 + c] = (7.5 + kap) * v[p * 4 + c];
+        for (int ax = 0; ax < 3; ++ax)
+          for (int d = -2; d <= 2; ++d) {
+            if (d == 0 || pos[ax] + d < 0 || pos[ax] + d >= n) continue;
+            const double cf = (d == 1 || d == -1) ? -16.0 / 12.0 : 1.0 / 12.0;
+            const size_t q = p + d * (long)st[ax];
+            for (int c = 0; c < 4; ++c) out[p * 4 + c] += cf * v[q * 4 + c];
+          }
+      }
+  (void)N3;
+}
+
+struct Fwd {
+  std::vector<double> a[5];  // a[0] input (8 ch), a[1..3] activations (16), a[4] output (4)
+};
+static void forward(int n, const float* in8, const std::vector<double>& w, Fwd& F) {
+  const size_t N3 = (size_t)n * n * n;
+  F.a[0].assign(in8, in8 + N3 * 8);
+  int cis = 8;
+  for (int l = 0; l < L; ++l) {
+    F.a[l + 1].assign(N3 * CO[l], 0.0);
+    conv(n, F.a[l].data(), cis, CI[l], w.data() + loff(l), CO[l], F.a[l + 1].data());
+    if (l < L - 1)
+      for (double& v : F.a[l + 1]) v = v > 0 ? v : SLOPE * v;
+    cis = CO[l];
+  }
+}
+}  // namespace lcp
+
+int orc_lcp_num_params() { return lcp::loff(lcp::L); }
+
+int orc_lcp_forward(int n, int batch, const float* in8, const double* params, double* out4) {
+  const size_t N3 = (size_t)n * n * n;
+  std::vector<double> w(params, params + orc_lcp_num_params());
+  for (int b = 0; b < batch; ++b) {
+    lcp::Fwd F;
+    lcp::forward(n, in8 + (size_t)b * N3 * 8, w, F);
+    std::copy(F.a[4].begin(), F.a[4].end(), out4 + (size_t)b * N3 * 4);
+  }
+  return 0;
+}
+
+int orc_lcp_loss_grad(int n, int batch, const float* in8, const float* tgt4, const double* params, double* loss,
+                      double* grad) {
+  using namespace lcp;
+  const size_t N3 = (size_t)n * n * n;
+  const int np = orc_lcp_num_params();
+  std::vector<double> w(params, params + np);
+  std::fill(grad, grad + np, 0.0);
+  *loss = 0.0;
+  for (int b = 0; b < batch; ++b) {
+    Fwd F;
+    forward(n, in8 + (size_t)b * N3 * 8, w, F);
+    const float* t = tgt4 + (size_t)b * N3 * 4;
+    const float* x = in8 + (size_t)b * N3 * 8;
+    std::vector<double> tv(t, t + 4 * N3), at(4 * N3), d(4 * N3), e(4 * N3);
+    aop(n, tv.data(), x, at.data());
+    double t2 = 0.0;
+    for (double v : at) t2 += v * v;
+    const double wt = 1.0 / (batch * t2);
+    for (size_t i = 0; i < 4 * N3; ++i) d[i] = F.a[4][i] - t[i];
+    aop(n, d.data(), x, e.data());
+    for (double v : e) *loss += wt * v * v;
+    std::vector<double> gz(4 * N3);  // dL / d(pre-activation of the current layer output)
+    aop(n, e.data(), x, gz.data());
+    for (double& v : gz) v *= 2.0 * wt;
+    for (int l = L - 1; l >= 0; --l) {
+      const int cin = CI[l], cout = CO[l], cis = l == 0 ? 8 : 16;
+      const double* in = F.a[l].data();
+      double* gw = grad + loff(l);
+      double* gb = gw + 27 * cin * cout;
+      std::vector<double> gin(N3 * cis, 0.0);
+      for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < n; ++i) {
+            const size_t p = i + (size_t)n * (j + (size_t)n * k);
+            const double* g = gz.data() + p * cout;
+            for (int co = 0; co < cout; ++co) gb[co] += g[co];
+            for (int dz = -1; dz <= 1; ++dz)
+              for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                  const int ii = i + dx, jj = j + dy, kk = k + dz;
+                  if (ii < 0 || jj < 0 || kk < 0 || ii >= n || jj >= n || kk >= n) continue;
+                  const int tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                  const size_t q = ii + (size_t)n * (jj + (size_t)n * kk);
+                  for (int ci = 0; ci < cin; ++ci)
+                    for (int co = 0; co < cout; ++co) {
+                      gw[(tap * cin + ci) * cout + co] += in[q * cis + ci] * g[co];
+                      gin[q * cis + ci] += w[loff(l) + (tap * cin + ci) * cout + co] * g[co];
+                    }
+                }
+          }
+      if (l == 0) break;
+      // through the LeakyReLU of the input activation
+      gz.assign(N3 * 16, 0.0);
+      for (size_t i = 0; i < N3 * 16; ++i) gz[i] = gin[i] * (in[i] > 0 ? 1.0 : SLOPE);
+    }
+  }
+  return 0;
+}
+
+// sizes of the boundary structs as this compiler lays them out (checks the
+// ctypes mirrors)
+int orc_sizeof_config() { return (int)sizeof(nhf_config); }
+int orc_sizeof_result() { return (int)sizeof(nhf_result); }
+
 }  // extern "C"

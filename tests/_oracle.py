@@ -198,3 +198,44 @@ def fields_once(cfg: NhfConfig, psi_n, psi_p, eps_n, eps_p):
                           dp(np.ascontiguousarray(eps_n, dtype=np.float64)),
                           dp(np.ascontiguousarray(eps_p, dtype=np.float64)), dp(f), dp(gf), dp(U), dp(W), dp(terms))
     return f.reshape(2, N3), gf.reshape(2, 3 * N3), U.reshape(2, N3), W.reshape(2, 3 * N3), terms
+
+
+def lcp_num_params():
+    return lib().orc_lcp_num_params()
+
+
+def lcp_forward(n, x8, params):
+    """x8: (batch, n^3, 8) float32, params: float64 -> (batch, n^3, 4) float64."""
+    x8 = np.ascontiguousarray(x8, dtype=np.float32)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros((x8.shape[0], n ** 3, 4))
+    lib().orc_lcp_forward(n, x8.shape[0], x8.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), dp(params), dp(out))
+    return out
+
+
+def lcp_loss_grad(n, x8, t4, params):
+    x8 = np.ascontiguousarray(x8, dtype=np.float32)
+    t4 = np.ascontiguousarray(t4, dtype=np.float32)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    loss = ctypes.c_double()
+    g = np.zeros(params.size)
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib().orc_lcp_loss_grad(n, x8.shape[0], x8.ctypes.data_as(fp), t4.ctypes.data_as(fp), dp(params),
+                            ctypes.byref(loss), dp(g))
+    return loss.value, g
+
+
+def lcp_random_case(n=5, batch=2, seed=3):
+    """Random net inputs (4 residual channels, kappa, 3 zero pads), targets
+    and He-scaled parameters of the net's shape."""
+    rng = np.random.default_rng(seed)
+    x8 = np.zeros((batch, n ** 3, 8), dtype=np.float32)
+    x8[:, :, :4] = rng.standard_normal((batch, n ** 3, 4))
+    x8[:, :, 4] = rng.uniform(-0.3, 0.6, (batch, n ** 3))
+    t4 = rng.standard_normal((batch, n ** 3, 4)).astype(np.float32)
+    ci, co = [5, 16, 16, 16], [16, 16, 16, 4]
+    parts = []
+    for l in range(4):
+        parts.append(rng.standard_normal(27 * ci[l] * co[l]) * np.sqrt(2.0 / (27 * ci[l])))
+        parts.append(rng.standard_normal(co[l]) * 0.1)
+    return x8, t4, np.concatenate(parts)

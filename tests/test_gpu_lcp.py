@@ -1,0 +1,110 @@
+"""The learned convolutional preconditioner on the GPU: its FP32 training
+kernels against the FP64 oracle net (forward, loss and parameter gradient),
+the BF16 MFMA inference kernels against the FP32 kernels, training on pairs
+the solver collects, and the ablation -- the same W-block budget with the
+trained net, with the plain Jacobi preconditioner, and with an untrained net."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import bench
+from nuclearcgm_amd import config as C
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from nuclearcgm_amd import hip
+
+    c = hip.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("n,batch", [(5, 2), (6, 3)])
+def test_lcp_kernels_match_oracle(ctx, n, batch):
+    x8, t4, w = O.lcp_random_case(n=n, batch=batch, seed=11)
+    w32 = w.astype(np.float32)
+    ctx.lcp_set_params(w32)
+    assert np.array_equal(ctx.lcp_params(), w32)
+    y = ctx.lcp_forward(x8, n)
+    y0 = O.lcp_forward(n, x8, w32.astype(np.float64))
+    assert np.abs(y - y0).max() < 1e-5 * np.abs(y0).max()
+    loss, g = ctx.lcp_loss_grad(x8, t4, n)
+    loss0, g0 = O.lcp_loss_grad(n, x8, t4, w32.astype(np.float64))
+    # FP32 kernels vs FP64 backpropagation (itself pinned by finite differences)
+    assert abs(loss - loss0) < 1e-5 * loss0
+    assert np.linalg.norm(g - g0) < 1e-5 * np.linalg.norm(g0)
+    offs = np.cumsum([0] + [27 * a * b + b for a, b in zip([5, 16, 16, 16], [16, 16, 16, 4])])
+    for l in range(4):  # every layer, not only the largest
+        s = slice(offs[l], offs[l + 1])
+        assert np.linalg.norm(g[s] - g0[s]) < 1e-4 * np.linalg.norm(g0[s])
+
+
+@pytest.mark.parametrize("n", [6, 17, 40])
+def test_lcp_bf16_inference_matches_fp32(ctx, n):
+    """BF16 activations and weights with FP32 accumulation: 0.5 % relative
+    (n = 6, 17 leave partial 8 x 8 x 4 bricks)."""
+    ctx.lcp_init(77)
+    rng = np.random.default_rng(n)
+    r = rng.standard_normal((3, n ** 3, 4)).astype(np.float32)
+    kap = rng.uniform(-0.3, 0.6, n ** 3).astype(np.float32)
+    hf = rng.uniform(0.01, 0.02, n ** 3).astype(np.float32)
+    zb = ctx.lcp_apply(r, kap, hf, n)
+    zf = ctx.lcp_apply(r, kap, hf, n, fp32=True)
+    assert np.linalg.norm(zb - zf) < 1.5e-2 * np.linalg.norm(zf)
+    # the net is applied to r / rms(r) and scaled back: positively homogeneous
+    z2 = ctx.lcp_apply(4.0 * r, kap, hf, n)
+    assert np.abs(z2 - 4.0 * zb).max() < 1e-5 * np.abs(zb).max()
+
+
+def test_lcp_training_and_ablation(ctx):
+    # training pairs from the plain solver on O-16 (30^3), training on the GPU
+    ctx.lcp_init(1234)
+    ctx.lcp_clear_pairs()
+    co = C.workload("o16", precision=C.PREC_MIXED)
+    co.max_iter = 80
+    ctx.lcp_collect(240, every=1)
+    ctx.problem(co).scf_run(*bench.make_guess(co))
+    ctx.lcp_collect(0)
+    assert ctx.lcp_num_pairs() == 240
+    first, _ = ctx.lcp_train(600, batch=4, lr=2e-3, seed=7)
+    _, last = ctx.lcp_train(600, batch=4, lr=5e-4, seed=8)
+    print(f"training loss {first:.3f} -> {last:.3f}")
+    assert last < 0.05 * first and last < 0.3
+    trained = ctx.lcp_params()
+    # ablation on Ca-40 (40^3, not seen in training): one W-block step each
+    cfg = C.workload("ca40", precision=C.PREC_MIXED)
+    g = bench.make_guess(cfg)
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ca40_oracle.json")))["E_int"]
+
+    def run(wstep, steps, iters=1):
+        c = C.workload("ca40", precision=C.PREC_MIXED)
+        c.max_iter = 300
+        c.mixing = C.MIX_BROYDEN
+        c.wstep, c.wstep_iters, c.gcg.steps = wstep, iters, steps
+        p = ctx.problem(c)
+        res = p.scf_run(*g)
+        _, k = bench.time_to_window(res[5], gold)
+        return res[4], k, p.lcp_stats()
+
+    r_l, k_l, s_l = run(C.W_LEARNED, 80)
+    r_j, k_j, _ = run(C.W_JACOBI_CG, 1)
+    ctx.lcp_init(1234)
+    r_u, k_u, s_u = run(C.W_LEARNED, 80)
+    ctx.lcp_set_params(trained)
+    print(f"iterations to 1 keV: learned {k_l} (ratio {s_l['n']['mean_ratio']:.3f}, fallbacks "
+          f"{s_l['n']['fallbacks']}), Jacobi one step {k_j}, untrained net {k_u} (ratio "
+          f"{s_u['n']['mean_ratio']:.3f}, fallbacks {s_u['n']['fallbacks']})")
+    for r in (r_l, r_j, r_u):  # every variant reaches the same fixed point
+        assert r.converged and abs(r.terms[C.E_INT] - gold) < 1e-3
+    # the trained net reduces the W-block residual; the untrained one does not
+    for q in "np":
+        assert s_l[q]["mean_ratio"] < 0.8 and s_l[q]["fallbacks"] <= 0.1 * s_l[q]["learned"]
+        assert s_u[q]["mean_ratio"] > 0.9
+    # and the SCF needs far fewer iterations than with one Jacobi step
+    assert 0 < k_l < 0.5 * k_j

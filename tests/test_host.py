@@ -1,0 +1,99 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every function include/nhf.h declares, the ctypes mirrors have the C layout,
+the host-only entry points (FP64 Rayleigh-Ritz eigensolver, Cholesky-QR
+coefficients, oscillator guess, EDF coefficients) agree with numpy and the
+oracle, and the InputParser mirror follows the reference's defaults."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from nuclearcgm_amd import config as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "nhf.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\**\s*(nhf_\w+)\s*\(", src, flags=re.M)))
+
+
+def test_library_exports_every_declared_function():
+    from nuclearcgm_amd import hip
+
+    names = declared_functions()
+    assert len(names) >= 25
+    lib = ctypes.CDLL(hip.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_mirrors_match_c_layout():
+    assert ctypes.sizeof(C.NhfConfig) == O.lib().orc_sizeof_config()
+    assert ctypes.sizeof(C.NhfResult) == O.lib().orc_sizeof_result()
+
+
+def test_host_eigh_lower_matches_numpy():
+    from nuclearcgm_amd import hip
+
+    rng = np.random.default_rng(4)
+    for k in (1, 7, 40, 96):
+        A = rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k))
+        # the strict lower triangle and the real diagonal define the matrix
+        H = np.tril(A, -1) + np.tril(A, -1).conj().T + np.diag(A.diagonal().real)
+        w, V = hip.eigh_lower(A)
+        w0 = np.linalg.eigvalsh(H)
+        assert np.abs(w - w0).max() < 1e-11 * max(1.0, np.abs(w0).max())
+        assert np.abs(V.conj().T @ V - np.eye(k)).max() < 1e-11
+        assert np.abs(H @ V - V * w).max() < 1e-10 * max(1.0, np.abs(w0).max())
+
+
+def test_host_cholqr_orthonormalises_and_drops_dependent_columns():
+    from nuclearcgm_amd import hip
+
+    rng = np.random.default_rng(5)
+    V = rng.standard_normal((200, 12)) + 1j * rng.standard_normal((200, 12))
+    V[:, 7] = 2 * V[:, 2] - 1j * V[:, 4]  # linearly dependent column
+    keep, Ci = hip.cholqr_coeffs(V.conj().T @ V)
+    assert 7 not in keep and len(keep) == 11
+    Q = V @ Ci
+    assert np.abs(Q.conj().T @ Q - np.eye(len(keep))).max() < 1e-10
+
+
+@pytest.mark.parametrize("n,nev", [(10, 8), (16, 20)])
+def test_host_ho_guess_matches_oracle(n, nev):
+    from nuclearcgm_amd import hip
+
+    b = C.ho_length(16)
+    assert np.abs(hip.ho_guess(n, 18.0, nev, b) - O.ho_guess(n, 18.0, nev, b)).max() < 1e-12
+
+
+def test_host_edf_matches_oracle():
+    from nuclearcgm_amd import hip
+
+    sk = np.array([-2488.91, 486.82, -546.39, 13777.0, 0.834, -0.344, -1.0, 1.354, 123.0, 123.0, 0.0, 0.0,
+                   1.0 / 6.0])
+    assert np.abs(hip.edf_from_skyrme(sk) - O.edf_from_skyrme(sk)).max() < 1e-12
+
+
+def test_input_parser_defaults():
+    d = {"functional": "SLy4", "maxIterations": 10, "energyTol": 1e-6,
+         "box": {"boxSize": 18.0, "axisMeshPoints": 30}, "nucleus": {"A": 16, "Z": 8}}
+    cfg = C.config_from_input(d)
+    assert (cfg.n, cfg.box_size, cfg.A, cfg.Z, cfg.M_n, cfg.M_p) == (30, 18.0, 16, 8, 8, 8)
+    assert cfg.density_mix == 0.25 and cfg.coulomb == 1 and cfg.start_hfb_iter == 3
+    assert cfg.use_j2 == 0 and cfg.spin_orbit == 1  # SLy4 functional file sets J2 false
+    assert (cfg.gcg.max_iter, cfg.gcg.steps, cfg.gcg.tol, cfg.gcg.cg_tol) == (1, 80, 1e-6, 1e-5)
+    assert cfg.mixing == C.MIX_LINEAR and cfg.wstep == C.W_JACOBI_CG  # the reference iteration
+    d["pairing"] = {"type": "BCS", "V0": 250.0, "window": 5.0, "HFbasisSize": 12}
+    cfg = C.config_from_input(d)
+    assert cfg.pairing_type == C.PAIR_BCS and (cfg.M_n, cfg.M_p) == (12, 12)
+    assert cfg.pair_n.V0 == 250.0 and cfg.pair_n.window_both == 1
+    assert C.check(cfg)
+    cfg.n = 31
+    assert not C.check(cfg)

@@ -335,3 +335,51 @@ def test_edf_nuclear_matter(name, rho0, ea, mstar, asym):
     # symmetry energy: kinetic + C1rr rho + C1Drr rho^(1+s) + C1rt-type terms
     esym = hb * kf ** 2 / 3 + Cc[1] * rho0 + Cc[3] * rho0 ** (1 + Cc[12]) + (Cc[4] / 3 + Cc[5]) * kf ** 2 * rho0
     assert abs(esym - asym) < 1.0
+
+
+# ------------------------------------------------ learned preconditioner net
+def test_lcp_oracle_gradient_matches_finite_differences():
+    """Pins the oracle's backpropagation: central differences of the oracle's
+    own loss at random parameters (all layers, weights and biases)."""
+    n = 4
+    x8, t4, w = O.lcp_random_case(n=n, batch=2, seed=5)
+    loss, g = O.lcp_loss_grad(n, x8, t4, w)
+    rng = np.random.default_rng(0)
+    offs = np.cumsum([0] + [27 * a * b + b for a, b in zip([5, 16, 16, 16], [16, 16, 16, 4])])
+    idx = list(rng.choice(w.size, 40, replace=False)) + [int(o) - 1 for o in offs[1:]]  # + last bias of each layer
+    for i in idx:
+        e = 1e-6 * max(1.0, abs(w[i]))
+        wp, wm = w.copy(), w.copy()
+        wp[i] += e
+        wm[i] -= e
+        fd = (O.lcp_loss_grad(n, x8, t4, wp)[0] - O.lcp_loss_grad(n, x8, t4, wm)[0]) / (2 * e)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-8, (i, fd, g[i])
+
+
+def test_lcp_oracle_forward_layout():
+    """Pins the tap and channel conventions: a net that passes input channel
+    c through the tap (dx, dy, dz) = (+1, 0, 0) in layer 0 and the centre tap
+    elsewhere returns in(i + 1, j, k)[c] (zero at the i = n-1 face) for
+    positive inputs; biases add through the identity chain."""
+    n = 5
+    ci, co = [5, 16, 16, 16], [16, 16, 16, 4]
+    w = []
+    for l in range(4):
+        W = np.zeros((27, ci[l], co[l]))
+        tap = (0 + 1) * 9 + (0 + 1) * 3 + (1 + 1) if l == 0 else 13
+        for c in range(4):
+            W[tap, c, c] = 1.0
+        w += [W.reshape(-1), np.zeros(co[l])]
+    w = np.concatenate(w)
+    rng = np.random.default_rng(1)
+    x8 = np.zeros((1, n ** 3, 8), dtype=np.float32)
+    x8[0, :, :4] = rng.uniform(0.1, 1.0, (n ** 3, 4))
+    y = O.lcp_forward(n, x8, w)[0].reshape(n, n, n, 4)   # (k, j, i, c)
+    x = x8[0, :, :4].reshape(n, n, n, 4).astype(np.float64)
+    exp = np.zeros_like(x)
+    exp[:, :, :-1, :] = x[:, :, 1:, :]
+    assert np.abs(y - exp).max() < 1e-6
+    # negative inputs go through three LeakyReLU(0.1): factor 1e-3
+    x8[0, :, :4] *= -1
+    y = O.lcp_forward(n, x8, w)[0].reshape(n, n, n, 4)
+    assert np.abs(y + 1e-3 * exp).max() < 1e-9
