@@ -260,6 +260,16 @@ int nhf_stencil_stats(void *ctx, int pid, int enable, double *out);
  * Kramers order / conjugate direction.  No reference counterpart. */
 int nhf_phase_times(void *ctx, int pid, int reset, double *out);
 
+/* The block product of the GCG (Gram / projection, solver.cpp:766-767 and
+ * the MGS inner products of :389-416) on host data: G = A^H B for complex
+ * spinor blocks of npts points (2 npts complex rows, ka and kb columns),
+ * with the device kernel of precision prec (NHF_PREC_MIXED: the data are
+ * rounded to FP32 and multiplied with split-BF16 MFMA products).  lower = 1
+ * (only with B == A) computes the lower block triangle and mirrors it.
+ * G: ka x kb complex, column-major. */
+int nhf_block_gram(void *ctx, size_t npts, int ka, const double *A, int kb, const double *B, int prec, int lower,
+                   double *G);
+
 /* Harmonic-oscillator spinor guess |n l j m_j> with oscillator length b
  * (fm), Euclidean-normalised columns (guess.cpp:10-248 with
  * useSpinOrbit = true).  Host-side; out: 2n^3 x nev complex. */

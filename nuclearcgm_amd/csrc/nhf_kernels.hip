@@ -12,10 +12,21 @@
 #include <cfloat>
 #include <cstdlib>
 #include <string>
+#include <type_traits>
 
 namespace nhf {
 
 constexpr int BS = 256;
+
+// FP32 block products: split-BF16 MFMA Gram and FP32-accumulating rotation
+// (default) or the FP64-accumulating kernels (environment NHF_BLOCK=fp64)
+static bool block_fp64() {
+  static bool v = [] {
+    const char* e = getenv("NHF_BLOCK");
+    return e && std::string(e) == "fp64";
+  }();
+  return v;
+}
 
 // stencil variant: 1 = 2.5D LDS/register kernel (default), 0 = direct loads
 // (environment NHF_HPSI=direct)
@@ -481,7 +492,9 @@ constexpr int GT = 16;
 constexpr int GRAM_CHUNK = 1024;
 constexpr int GROWS = 32;  // points per LDS stage
 
-int gram_chunks(size_t N3) { return (int)((N3 + GRAM_CHUNK - 1) / GRAM_CHUNK); }
+// partial-sum slots the Gram launches may use: chunks of GRAM_CHUNK points
+// (FP64 kernel) or of at least 256 points (split MFMA kernel)
+int gram_chunks(size_t N3) { return (int)((N3 + 255) / 256); }
 
 template <typename R>
 __global__ __launch_bounds__(BS) void k_gram(const Sp<R>* __restrict__ A, size_t lda, int ka,
@@ -526,14 +539,15 @@ __global__ __launch_bounds__(BS) void k_gram(const Sp<R>* __restrict__ A, size_t
   if (i < ka && j < kb) part[((size_t)chunk * kb + j) * ka + i] = make_double2(sr, si);
 }
 
-// sums the chunk partials; with `lower` (Hermitian result) the entries of
-// skipped upper tiles are taken as the conjugate of their mirror
+// sums the chunk partials; with a lower-triangle tile edge `lower` > 0
+// (Hermitian result) the entries of skipped upper tiles are taken as the
+// conjugate of their mirror
 __global__ void k_reduce_parts(const double2* __restrict__ part, int nchunk, size_t cnt, int ka, int lower,
                                double2* __restrict__ C) {
   size_t e = (size_t)blockIdx.x * BS + threadIdx.x;
   if (e >= cnt) return;
   const int i = (int)(e % ka), j = (int)(e / ka);
-  const bool mirror = lower && (i / GT) < (j / GT);
+  const bool mirror = lower > 0 && (i / lower) < (j / lower);
   const size_t src = mirror ? (size_t)i * ka + j : e;
   double sr = 0.0, si = 0.0;
   for (int c = 0; c < nchunk; ++c) {
@@ -544,15 +558,187 @@ __global__ void k_reduce_parts(const double2* __restrict__ part, int nchunk, siz
   C[e] = mirror ? make_double2(sr, -si) : make_double2(sr, si);
 }
 
+// ---- split-precision MFMA Gram of FP32 states (v_mfma_f32_16x16x16_bf16).
+// Each FP32 operand is split three ways, x = h + m + l with h = bf16(x),
+// m = bf16(x - h), l = bf16(x - h - m) (about 24 significant bits, the FP32
+// level); a product is hh' + hm' + mh' + hl' + lh' + mm' (the dropped terms
+// are below 2^-24 relative), accumulated in FP32 over a chunk of points and
+// in FP64 across chunks.  A two-way split (17 bits) is not enough: the
+// projected matrix V^H (H + sigma) V has entries of O(10^3) MeV for the
+// high-frequency W-block columns, and 1e-5 relative errors there move the
+// Ritz values by O(10^-2) MeV.  Over the 4 n^3
+// real components, Re(A^H B) = A . B and Im(A^H B) = A . B' with B' = -i B
+// (component pairs (re, im) -> (im, -re)).  A wavefront owns a 32 x 32 output
+// block (2 x 2 MFMA tiles) over a quarter of the workgroup's chunk; the four
+// wavefronts are summed through LDS.  MFMA operand layout: lane l holds
+// A[l % 16][4 (l / 16) + r] and B[4 (l / 16) + r][l % 16] (here: component r
+// of point 4 s + l / 16 of column l % 16) and D[4 (l / 16) + r][l % 16].
+// The FP32 MFMA accumulators are added into FP64 sums every GFLUSH slabs, so
+// the result is accurate to ~1e-8 relative (the limit of the split), not to
+// the ~1e-7 of a long FP32 sum.
+typedef short gbf16x4 __attribute__((ext_vector_type(4)));
+typedef float gf32x4 __attribute__((ext_vector_type(4)));
+constexpr int GB = 32;     // output block edge
+constexpr int GFLUSH = 4;  // slabs (16 points) per FP32 partial sum
+__device__ __forceinline__ unsigned short g_f2bf(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float g_bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ void split4(const Sp<float>& v, gbf16x4& hi, gbf16x4& mi, gbf16x4& lo) {
+  const float x[4] = {v.ur, v.ui, v.dr, v.di};
+  unsigned short h[4], m[4], l[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    h[r] = g_f2bf(x[r]);
+    const float r1 = x[r] - g_bf2f(h[r]);
+    m[r] = g_f2bf(r1);
+    l[r] = g_f2bf(r1 - g_bf2f(m[r]));
+  }
+  hi = gbf16x4{(short)h[0], (short)h[1], (short)h[2], (short)h[3]};
+  mi = gbf16x4{(short)m[0], (short)m[1], (short)m[2], (short)m[3]};
+  lo = gbf16x4{(short)l[0], (short)l[1], (short)l[2], (short)l[3]};
+}
+// (re, im, re, im) -> (im, -re, im, -re): multiplication by -i
+__device__ __forceinline__ gbf16x4 mul_mi(const gbf16x4& v) {
+  return gbf16x4{v[1], (short)(v[0] ^ (short)0x8000), v[3], (short)(v[2] ^ (short)0x8000)};
+}
+__global__ __launch_bounds__(BS) void k_gram_mfma(const Sp<float>* __restrict__ A, size_t lda, int ka,
+                                                  const Sp<float>* __restrict__ B, size_t ldb, int kb, size_t N3,
+                                                  int chunk_pts, double2* __restrict__ part, int lower) {
+  __shared__ double red[BS / 64 - 1][8][4][64];
+  const int nbj = (kb + GB - 1) / GB;
+  const int i0 = (blockIdx.x / nbj) * GB, j0 = (blockIdx.x % nbj) * GB;
+  if (lower && i0 < j0) return;  // block strictly above the diagonal
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+  const size_t p0 = (size_t)blockIdx.y * chunk_pts;
+  const size_t p1 = min(N3, p0 + (size_t)chunk_pts);
+  const int ia[2] = {i0 + c, i0 + 16 + c}, jb[2] = {j0 + c, j0 + 16 + c};
+  gf32x4 re[2][2], im[2][2];
+  double dre[2][2][4], dim[2][2][4];  // FP64 running sums, flushed every GFLUSH slabs
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      re[a][b] = gf32x4{0.f, 0.f, 0.f, 0.f};
+      im[a][b] = gf32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dre[a][b][r] = dim[a][b][r] = 0.0;
+    }
+  auto flush = [&]() {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          dre[a][b][r] += re[a][b][r];
+          dim[a][b][r] += im[a][b][r];
+        }
+        re[a][b] = gf32x4{0.f, 0.f, 0.f, 0.f};
+        im[a][b] = gf32x4{0.f, 0.f, 0.f, 0.f};
+      }
+  };
+  // a slab = 4 points; wavefront w takes slabs w, w + 4, ...
+  int ns = 0;
+  for (size_t base = p0 + 4 * (size_t)wave; base < p1; base += 16) {
+    if (++ns == GFLUSH) {
+      flush();
+      ns = 0;
+    }
+    const size_t p = base + g;
+    gbf16x4 ah[2], am[2], al[2], bh[2], bm[2], bl[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      Sp<float> va{0.f, 0.f, 0.f, 0.f}, vb{0.f, 0.f, 0.f, 0.f};
+      if (p < p1 && ia[a] < ka) va = A[(size_t)ia[a] * lda + p];
+      if (p < p1 && jb[a] < kb) vb = B[(size_t)jb[a] * ldb + p];
+      split4(va, ah[a], am[a], al[a]);
+      split4(vb, bh[a], bm[a], bl[a]);
+    }
+#define GMF(acc, x, y) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, y, acc, 0, 0, 0)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        // smallest terms first
+        GMF(re[a][b], am[a], bm[b]);
+        GMF(re[a][b], al[a], bh[b]);
+        GMF(re[a][b], ah[a], bl[b]);
+        GMF(re[a][b], am[a], bh[b]);
+        GMF(re[a][b], ah[a], bm[b]);
+        GMF(re[a][b], ah[a], bh[b]);
+        const gbf16x4 bhx = mul_mi(bh[b]), bmx = mul_mi(bm[b]), blx = mul_mi(bl[b]);
+        GMF(im[a][b], am[a], bmx);
+        GMF(im[a][b], al[a], bhx);
+        GMF(im[a][b], ah[a], blx);
+        GMF(im[a][b], am[a], bhx);
+        GMF(im[a][b], ah[a], bmx);
+        GMF(im[a][b], ah[a], bhx);
+      }
+#undef GMF
+  }
+  flush();
+  // sum the four wavefronts
+  if (wave > 0)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          red[wave - 1][2 * (2 * a + b)][r][lane] = dre[a][b][r];
+          red[wave - 1][2 * (2 * a + b) + 1][r][lane] = dim[a][b][r];
+        }
+  __syncthreads();
+  if (wave != 0) return;
+  double2* out = part + (size_t)blockIdx.y * ka * kb;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double sr = dre[a][b][r], si = dim[a][b][r];
+        for (int w = 0; w < BS / 64 - 1; ++w) {
+          sr += red[w][2 * (2 * a + b)][r][lane];
+          si += red[w][2 * (2 * a + b) + 1][r][lane];
+        }
+        const int i = i0 + 16 * a + 4 * g + r, j = j0 + 16 * b + c;
+        if (i < ka && j < kb) out[(size_t)j * ka + i] = make_double2(sr, si);
+      }
+}
+
+// chunk length (points) for the split Gram: enough workgroups to fill the
+// device (about 1024), a multiple of 16, at least 256 points
+static int gram_mfma_chunk(size_t N3, int blocks) {
+  size_t want = (N3 * (size_t)blocks + 1023) / 1024;
+  want = (want + 15) / 16 * 16;
+  return (int)std::max<size_t>(256, want);
+}
+
 template <typename R>
 void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>* B, size_t ldb, int kb, size_t N3,
                  double2* part, int nchunk, double2* C, int lower) {
-  int nti = (ka + GT - 1) / GT, ntj = (kb + GT - 1) / GT;
   lower = (lower && A == B && ka == kb) ? 1 : 0;
-  hipLaunchKernelGGL((k_gram<R>), dim3(nti * ntj, nchunk), dim3(BS), 0, st, A, lda, ka, B, ldb, kb, N3, part, lower);
   size_t cnt = (size_t)ka * kb;
-  hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((cnt + BS - 1) / BS)), dim3(BS), 0, st, part, nchunk, cnt, ka,
-                     lower, C);
+  if (std::is_same<R, float>::value && !block_fp64()) {
+    const int nbi = (ka + GB - 1) / GB, nbj = (kb + GB - 1) / GB;
+    const int cp = gram_mfma_chunk(N3, nbi * nbj);
+    const int nc = (int)((N3 + cp - 1) / cp);
+    hipLaunchKernelGGL(k_gram_mfma, dim3(nbi * nbj, nc), dim3(BS), 0, st, (const Sp<float>*)A, lda, ka,
+                       (const Sp<float>*)B, ldb, kb, N3, cp, part, lower);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((cnt + BS - 1) / BS)), dim3(BS), 0, st, part, nc, cnt, ka,
+                       lower ? GB : 0, C);
+    (void)nchunk;
+    return;
+  }
+  int nti = (ka + GT - 1) / GT, ntj = (kb + GT - 1) / GT;
+  const int ncd = std::min(nchunk, (int)((N3 + GRAM_CHUNK - 1) / GRAM_CHUNK));
+  hipLaunchKernelGGL((k_gram<R>), dim3(nti * ntj, ncd), dim3(BS), 0, st, A, lda, ka, B, ldb, kb, N3, part, lower);
+  hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((cnt + BS - 1) / BS)), dim3(BS), 0, st, part, ncd, cnt, ka,
+                     lower ? GT : 0, C);
 }
 template void launch_gram<double>(hipStream_t, const Sp<double>*, size_t, int, const Sp<double>*, size_t, int, size_t,
                                   double2*, int, double2*, int);
@@ -561,54 +747,68 @@ template void launch_gram<float>(hipStream_t, const Sp<float>*, size_t, int, con
 
 // Y[:, c] (+)= sum_l V[:, l] C[l, c]; optional residual norms
 // |AV c - w_c V c|^2 (the Ritz residual of column c).
-constexpr int RT = 8;  // output columns per workgroup
-template <typename R, bool RES>
+// The residual variant and FP64 states accumulate in FP64 (RT = 8 output
+// columns per workgroup); the plain rotation of FP32 states accumulates in
+// FP32 with 16 columns per workgroup (half the re-reads of V).
+constexpr int RT = 8;  // output columns per workgroup (FP64 accumulation)
+template <typename R, bool RES, typename Acc = double, int RTN = RT>
 __global__ __launch_bounds__(BS) void k_rotate(const Sp<R>* __restrict__ V, size_t ldv, int k,
                                                const double2* __restrict__ C, int ncols, Sp<R>* __restrict__ Y,
                                                size_t ldy, size_t N3, const Sp<R>* __restrict__ AV,
                                                const double* __restrict__ w, double* __restrict__ rn2, int accumulate) {
-  extern __shared__ double2 Cs[];  // k x RT
-  const int c0 = blockIdx.y * RT;
-  const int nc = min(RT, ncols - c0);
-  for (int e = threadIdx.x; e < k * RT; e += BS) {
-    const int l = e / RT, cc = e % RT;
-    Cs[e] = (cc < nc) ? C[(size_t)(c0 + cc) * k + l] : make_double2(0.0, 0.0);
+  using A2 = typename std::conditional<std::is_same<Acc, double>::value, double2, float2>::type;
+  extern __shared__ __align__(16) unsigned char rot_sh[];
+  A2* Cs = reinterpret_cast<A2*>(rot_sh);  // k x RTN
+  const int c0 = blockIdx.y * RTN;
+  const int nc = min(RTN, ncols - c0);
+  for (int e = threadIdx.x; e < k * RTN; e += BS) {
+    const int l = e / RTN, cc = e % RTN;
+    const double2 v = (cc < nc) ? C[(size_t)(c0 + cc) * k + l] : make_double2(0.0, 0.0);
+    Cs[e] = A2{(Acc)v.x, (Acc)v.y};
   }
   __syncthreads();
   const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
-  double res[RT];
+  double res[RTN];
 #pragma unroll
-  for (int cc = 0; cc < RT; ++cc) res[cc] = 0.0;
+  for (int cc = 0; cc < RTN; ++cc) res[cc] = 0.0;
   if (p < N3) {
-    double y[RT][4], ay[RT][4];
+    Acc y[RTN][4], ay[RES ? RTN : 1][4];
 #pragma unroll
-    for (int cc = 0; cc < RT; ++cc)
+    for (int cc = 0; cc < RTN; ++cc)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        y[cc][q] = 0.0;
-        ay[cc][q] = 0.0;
+        y[cc][q] = 0;
+        if (RES) ay[RES ? cc : 0][q] = 0;
       }
     for (int l = 0; l < k; ++l) {
-      const Sp<double> v = to_d(V[(size_t)l * ldv + p]);
-      Sp<double> av = {0, 0, 0, 0};
-      if (RES) av = to_d(AV[(size_t)l * ldv + p]);
+      const Sp<R> vr = V[(size_t)l * ldv + p];
+      const Acc v0 = (Acc)vr.ur, v1 = (Acc)vr.ui, v2 = (Acc)vr.dr, v3 = (Acc)vr.di;
+      Acc a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+      if (RES) {
+        const Sp<R> ar = AV[(size_t)l * ldv + p];
+        a0 = (Acc)ar.ur;
+        a1 = (Acc)ar.ui;
+        a2 = (Acc)ar.dr;
+        a3 = (Acc)ar.di;
+      }
 #pragma unroll
-      for (int cc = 0; cc < RT; ++cc) {
-        const double2 cf = Cs[l * RT + cc];
-        y[cc][0] += v.ur * cf.x - v.ui * cf.y;
-        y[cc][1] += v.ur * cf.y + v.ui * cf.x;
-        y[cc][2] += v.dr * cf.x - v.di * cf.y;
-        y[cc][3] += v.dr * cf.y + v.di * cf.x;
+      for (int cc = 0; cc < RTN; ++cc) {
+        const A2 cf = Cs[l * RTN + cc];
+        y[cc][0] += v0 * cf.x - v1 * cf.y;
+        y[cc][1] += v0 * cf.y + v1 * cf.x;
+        y[cc][2] += v2 * cf.x - v3 * cf.y;
+        y[cc][3] += v2 * cf.y + v3 * cf.x;
         if (RES) {
-          ay[cc][0] += av.ur * cf.x - av.ui * cf.y;
-          ay[cc][1] += av.ur * cf.y + av.ui * cf.x;
-          ay[cc][2] += av.dr * cf.x - av.di * cf.y;
-          ay[cc][3] += av.dr * cf.y + av.di * cf.x;
+          const int ci = RES ? cc : 0;
+          ay[ci][0] += a0 * cf.x - a1 * cf.y;
+          ay[ci][1] += a0 * cf.y + a1 * cf.x;
+          ay[ci][2] += a2 * cf.x - a3 * cf.y;
+          ay[ci][3] += a2 * cf.y + a3 * cf.x;
         }
       }
     }
 #pragma unroll
-    for (int cc = 0; cc < RT; ++cc) {
+    for (int cc = 0; cc < RTN; ++cc) {
       if (cc < nc) {
         Sp<R>* dst = Y + (size_t)(c0 + cc) * ldy + p;
         Sp<R> o;
@@ -624,7 +824,7 @@ __global__ __launch_bounds__(BS) void k_rotate(const Sp<R>* __restrict__ V, size
           double s = 0.0;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            double r = ay[cc][q] - wc * y[cc][q];
+            double r = (double)ay[RES ? cc : 0][q] - wc * (double)y[cc][q];
             s += r * r;
           }
           res[cc] = s;
@@ -632,12 +832,20 @@ __global__ __launch_bounds__(BS) void k_rotate(const Sp<R>* __restrict__ V, size
       }
     }
   }
-  if (RES) block_atomic_add<RT>(res, rn2 + c0);
+  if (RES) block_atomic_add<RTN>(res, rn2 + c0);
 }
 
 template <typename R>
 void launch_rotate(hipStream_t st, const Sp<R>* V, size_t ldv, int k, const double2* C, int ncols, Sp<R>* Y,
                    size_t ldy, size_t N3, const Sp<R>* AV, const double* w, double* rn2, int accumulate) {
+  if (std::is_same<R, float>::value && !AV && !block_fp64()) {
+    constexpr int RF = 16;
+    dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)((ncols + RF - 1) / RF));
+    size_t sh = (size_t)k * RF * sizeof(float2);
+    hipLaunchKernelGGL((k_rotate<R, false, float, RF>), grid, dim3(BS), sh, st, V, ldv, k, C, ncols, Y, ldy, N3, AV,
+                       w, rn2, accumulate);
+    return;
+  }
   dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)((ncols + RT - 1) / RT));
   size_t sh = (size_t)k * RT * sizeof(double2);
   if (AV)
@@ -675,13 +883,14 @@ template void launch_axpby_cols<double>(hipStream_t, const Sp<double>*, Sp<doubl
 template void launch_axpby_cols<float>(hipStream_t, const Sp<float>*, Sp<float>*, size_t, size_t, int, const double*,
                                        const double*);
 
+// out[c] += sum_p weight_p |X_c(p)|^2: at most 32 workgroups per column walk
+// the points (grid-stride), so each column takes few atomics
 template <typename R>
 __global__ __launch_bounds__(BS) void k_col_norms(const Sp<R>* __restrict__ X, size_t N3, size_t ld, int n, int trap,
                                                   double* __restrict__ out) {
   const int c = blockIdx.y;
-  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
   double v[1] = {0.0};
-  if (p < N3) {
+  for (size_t p = (size_t)blockIdx.x * BS + threadIdx.x; p < N3; p += (size_t)gridDim.x * BS) {
     Sp<R> x = X[(size_t)c * ld + p];
     double s = (double)x.ur * x.ur + (double)x.ui * x.ui + (double)x.dr * x.dr + (double)x.di * x.di;
     if (trap) {
@@ -689,14 +898,14 @@ __global__ __launch_bounds__(BS) void k_col_norms(const Sp<R>* __restrict__ X, s
       decompose(p, n, i, j, k);
       s *= trap_w(i, j, k, n);
     }
-    v[0] = s;
+    v[0] += s;
   }
   block_atomic_add<1>(v, out + c);
 }
 template <typename R>
 void launch_col_norms(hipStream_t st, const Sp<R>* X, size_t N3, size_t ld, int ncols, int n, int trapezoid,
                       double* out) {
-  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  dim3 grid((unsigned)std::min<size_t>((N3 + BS - 1) / BS, 32), (unsigned)ncols);
   hipLaunchKernelGGL((k_col_norms<R>), grid, dim3(BS), 0, st, X, N3, ld, n, trapezoid, out);
 }
 template void launch_col_norms<double>(hipStream_t, const Sp<double>*, size_t, size_t, int, int, int, double*);

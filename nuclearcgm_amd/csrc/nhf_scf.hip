@@ -12,6 +12,7 @@
 #include <numeric>
 #include <thread>
 #include <tuple>
+#include <type_traits>
 
 #include "nhf_internal.h"
 #include "nhf_lcp.h"
@@ -416,7 +417,15 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
   auto tA = Clock::now();
   // 1. X = orthonormalised guess, into V1[0:M]
   CHK(gram_host<R>(P, Wk, psi_in, M, psi_in, M, G));
-  int ke = cholqr_coeffs(M, G.data(), 1e-14, keep, C1);
+  // column-drop and second-pass thresholds of the Cholesky-QR: the FP64
+  // path follows the modified Gram-Schmidt + keep filter of the reference;
+  // the FP32-state path computes its Gram matrices to ~1e-8 relative (three-
+  // way split BF16 products), so a column whose residual norm^2 is below 1e-7
+  // of its norm^2 carries no reliable direction, and a first pass that met a
+  // ratio below 1e-4 leaves an orthogonality error that needs the second pass.
+  const bool fp32 = std::is_same<R, float>::value;
+  const double drop_rel = fp32 ? 1e-7 : 1e-14, pass2_rel = fp32 ? 1e-4 : 1e-8;
+  int ke = cholqr_coeffs(M, G.data(), drop_rel, keep, C1);
   if (ke < M) return NHF_ESOLVER;
   CHK(rotate_dev<R>(P, Wk, psi_in, M, C1, M, V1, nullptr, nullptr, false));
   // 2. Rayleigh-Ritz on X with shift 0
@@ -552,17 +561,17 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
     const int k = 2 * M + pc;
     launch_col_normalize<R>(st, V, N3, ld, k, P.n, 0, 1.0, Wk.norms);
     // 5. Cholesky-QR with column dropping; a second pass only when the basis
-    //    was ill conditioned (smallest new-direction norm^2 below 1e-8)
+    //    was ill conditioned (smallest new-direction norm^2 below pass2_rel)
     CHK(gram_host<R>(P, Wk, V, k, V, k, G));
     double mrel = 1.0;
-    int k1 = cholqr_coeffs(k, G.data(), 1e-14, keep, C1, &mrel);
+    int k1 = cholqr_coeffs(k, G.data(), drop_rel, keep, C1, &mrel);
     if (k1 < 1) return NHF_ESOLVER;
     CHK(rotate_dev<R>(P, Wk, V, k, C1, k1, V1, nullptr, nullptr, false));
     int k2 = k1;
-    bool pass2 = mrel < 1e-8;
+    bool pass2 = mrel < pass2_rel;
     if (pass2) {
       CHK(gram_host<R>(P, Wk, V1, k1, V1, k1, G));
-      k2 = cholqr_coeffs(k1, G.data(), 1e-14, keep, C2);  // k1 x k2
+      k2 = cholqr_coeffs(k1, G.data(), drop_rel, keep, C2);  // k1 x k2
       if (k2 < 1) return NHF_ESOLVER;
     }
     if (info) info->basis = k2;
@@ -1626,6 +1635,46 @@ int nhf_lcp_loss_grad(void* ctx, int n, int batch, const float* in8, const float
   HIPCHK(hipMemcpyAsync(grad, c->lcp.g, sizeof(float) * LCP_NPARAM, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   return NHF_OK;
+}
+
+int nhf_block_gram(void* ctx, size_t npts, int ka, const double* A, int kb, const double* B, int prec, int lower,
+                   double* G) {
+  if (!ctx || !A || !B || !G || npts < 1 || ka < 1 || kb < 1 || (prec != NHF_PREC_FP64 && prec != NHF_PREC_MIXED) ||
+      (lower && (A != B || ka != kb)))
+    return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t es = prec == NHF_PREC_FP64 ? sizeof(Sp<double>) : sizeof(Sp<float>);
+  const int nch = gram_chunks(npts);
+  void *da = nullptr, *db = nullptr, *dp = nullptr, *dg = nullptr, *tmp = nullptr;
+  int rc = NHF_OK;
+  if (hipMalloc(&da, es * npts * ka) != hipSuccess || hipMalloc(&db, es * npts * kb) != hipSuccess ||
+      hipMalloc(&dp, sizeof(double2) * (size_t)nch * ka * kb) != hipSuccess ||
+      hipMalloc(&dg, sizeof(double2) * (size_t)ka * kb) != hipSuccess ||
+      hipMalloc(&tmp, sizeof(Sp<double>) * npts * std::max(ka, kb)) != hipSuccess)
+    rc = NHF_ENOMEM;
+  auto up = [&](const double* h, void* d, int k) {
+    if (prec == NHF_PREC_FP64) return hipMemcpy(d, h, sizeof(Sp<double>) * npts * k, hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(tmp, h, sizeof(Sp<double>) * npts * k, hipMemcpyHostToDevice);
+    if (e == hipSuccess) launch_convert<double, float>(c->st, (const Sp<double>*)tmp, (Sp<float>*)d, npts * k);
+    return e == hipSuccess ? hipStreamSynchronize(c->st) : e;
+  };
+  if (rc == NHF_OK && (up(A, da, ka) != hipSuccess || up(B, db, kb) != hipSuccess)) rc = NHF_EHIP;
+  if (rc == NHF_OK) {
+    const void* bb = lower ? da : db;
+    if (prec == NHF_PREC_FP64)
+      launch_gram<double>(c->st, (const Sp<double>*)da, npts, ka, (const Sp<double>*)bb, npts, kb, npts,
+                          (double2*)dp, nch, (double2*)dg, lower);
+    else
+      launch_gram<float>(c->st, (const Sp<float>*)da, npts, ka, (const Sp<float>*)bb, npts, kb, npts,
+                         (double2*)dp, nch, (double2*)dg, lower);
+    if (hipStreamSynchronize(c->st) != hipSuccess ||
+        hipMemcpy(G, dg, sizeof(double2) * (size_t)ka * kb, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = NHF_EHIP;
+  }
+  for (void* q : {da, db, dp, dg, tmp})
+    if (q) (void)hipFree(q);
+  return rc;
 }
 
 int nhf_lcp_apply(void* ctx, int n, int m, const float* r, const float* kap, const float* hf, float* z, int mode) {

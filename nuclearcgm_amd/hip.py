@@ -48,6 +48,8 @@ def lib():
         L.nhf_phase_times.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         L.nhf_ho_guess.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]
         L.nhf_eigh_lower.argtypes = [ctypes.c_int, _D, _D, _D]
+        L.nhf_block_gram.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, _D, ctypes.c_int, _D,
+                                     ctypes.c_int, ctypes.c_int, _D]
         L.nhf_cholqr_coeffs.argtypes = [ctypes.c_int, _D, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _D]
         L.nhf_edf_from_skyrme.argtypes = [_D, _D]
         L.nhf_edf_from_skyrme.restype = None
@@ -167,6 +169,18 @@ class Context:
         pid = ctypes.c_int()
         _chk(lib().nhf_problem(self.h, ctypes.byref(cfg), ctypes.byref(pid)))
         return Problem(self, pid.value, cfg)
+
+    def block_gram(self, A, B=None, prec=PREC_FP64):
+        """G = A^H B with the library's device block product (A, B: complex
+        (2 npts, k) column blocks).  B = None: the Hermitian A^H A from the
+        lower block triangle."""
+        A = _cols(A)
+        lower = B is None
+        Bm = A if lower else _cols(B)
+        G = np.zeros((A.shape[1], Bm.shape[1]), dtype=np.complex128, order="F")
+        _chk(lib().nhf_block_gram(self.h, A.shape[0] // 2, A.shape[1], _dp(A), Bm.shape[1], _dp(Bm), prec,
+                                  int(lower), _dp(G)))
+        return G
 
     # ---- learned convolutional preconditioner (one per context)
     def lcp_init(self, seed):

@@ -101,3 +101,25 @@ def test_scf_o16_matches_oracle(H):
     assert np.abs(r_or[3] - r_gp[3]).max() < 1e-4
     assert abs(r_or[4].r2_n - r_gp[4].r2_n) < 1e-5
     assert abs(r_or[4].r2_p - r_gp[4].r2_p) < 1e-5
+
+
+@pytest.mark.parametrize("npts,ka,kb", [(1000, 5, 7), (27000, 40, 33), (64000, 96, 96)])
+def test_block_gram(H, npts, ka, kb):
+    """The GCG block product A^H B: FP64 kernel to rounding; split-BF16 MFMA
+    kernel on FP32 data to 3e-8 of |A||B| (three-way split, FP64 sums every
+    16 points), both for a general product and the Hermitian lower-block
+    variant."""
+    _, ctx = H
+    rng = np.random.default_rng(npts)
+    A = rng.standard_normal((2 * npts, ka)) + 1j * rng.standard_normal((2 * npts, ka))
+    B = rng.standard_normal((2 * npts, kb)) + 1j * rng.standard_normal((2 * npts, kb))
+    A /= np.linalg.norm(A, axis=0)
+    B /= np.linalg.norm(B, axis=0)
+    G0 = A.conj().T @ B
+    assert np.abs(ctx.block_gram(A, B, C.PREC_FP64) - G0).max() < 1e-14
+    A32 = A.astype(np.complex64).astype(np.complex128)
+    B32 = B.astype(np.complex64).astype(np.complex128)
+    assert np.abs(ctx.block_gram(A, B, C.PREC_MIXED) - A32.conj().T @ B32).max() < 3e-8
+    H = ctx.block_gram(A, None, C.PREC_MIXED)
+    assert np.abs(H - A32.conj().T @ A32).max() < 3e-8
+    assert np.abs(H - H.conj().T).max() == 0.0  # mirrored exactly
