@@ -88,6 +88,10 @@ void launch_col_scale(hipStream_t st, Sp<R>* X, size_t N3, size_t ld, int ncols,
 template <typename R>
 void launch_col_normalize(hipStream_t st, Sp<R>* X, size_t N3, size_t ld, int ncols, int n, int trapezoid, double w,
                           double* norms);
+// dst column c = src column order[c] (order: device array of ncols)
+template <typename R>
+void launch_gather_cols(hipStream_t st, const Sp<R>* src, Sp<R>* dst, const int* order, size_t N3, size_t ld,
+                        int ncols);
 template <typename Ra, typename Rb>
 void launch_convert(hipStream_t st, const Sp<Ra>* a, Sp<Rb>* b, size_t count);
 template <typename R>

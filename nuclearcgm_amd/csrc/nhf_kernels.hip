@@ -541,22 +541,34 @@ __global__ __launch_bounds__(BS) void k_gram(const Sp<R>* __restrict__ A, size_t
 
 // sums the chunk partials; with a lower-triangle tile edge `lower` > 0
 // (Hermitian result) the entries of skipped upper tiles are taken as the
-// conjugate of their mirror
-__global__ void k_reduce_parts(const double2* __restrict__ part, int nchunk, size_t cnt, int ka, int lower,
-                               double2* __restrict__ C) {
-  size_t e = (size_t)blockIdx.x * BS + threadIdx.x;
-  if (e >= cnt) return;
-  const int i = (int)(e % ka), j = (int)(e / ka);
-  const bool mirror = lower > 0 && (i / lower) < (j / lower);
-  const size_t src = mirror ? (size_t)i * ka + j : e;
+// conjugate of their mirror.  Workgroup: 32 entries x 8 groups of chunks.
+__global__ __launch_bounds__(BS) void k_reduce_parts(const double2* __restrict__ part, int nchunk, size_t cnt, int ka,
+                                                     int lower, double2* __restrict__ C) {
+  __shared__ double2 sh[BS / 32][32];
+  const int le = threadIdx.x % 32, cg = threadIdx.x / 32;
+  const size_t e = (size_t)blockIdx.x * 32 + le;
   double sr = 0.0, si = 0.0;
-  for (int c = 0; c < nchunk; ++c) {
-    double2 v = part[(size_t)c * cnt + src];
-    sr += v.x;
-    si += v.y;
+  bool mirror = false;
+  if (e < cnt) {
+    const int i = (int)(e % ka), j = (int)(e / ka);
+    mirror = lower > 0 && (i / lower) < (j / lower);
+    const size_t src = mirror ? (size_t)i * ka + j : e;
+    for (int c = cg; c < nchunk; c += BS / 32) {
+      const double2 v = part[(size_t)c * cnt + src];
+      sr += v.x;
+      si += v.y;
+    }
+  }
+  sh[cg][le] = make_double2(sr, si);
+  __syncthreads();
+  if (cg != 0 || e >= cnt) return;
+  for (int k = 1; k < BS / 32; ++k) {
+    sr += sh[k][le].x;
+    si += sh[k][le].y;
   }
   C[e] = mirror ? make_double2(sr, -si) : make_double2(sr, si);
 }
+static unsigned reduce_grid(size_t cnt) { return (unsigned)((cnt + 31) / 32); }
 
 // ---- split-precision MFMA Gram of FP32 states (v_mfma_f32_16x16x16_bf16).
 // Each FP32 operand is split three ways, x = h + m + l with h = bf16(x),
@@ -729,16 +741,15 @@ void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>
     const int nc = (int)((N3 + cp - 1) / cp);
     hipLaunchKernelGGL(k_gram_mfma, dim3(nbi * nbj, nc), dim3(BS), 0, st, (const Sp<float>*)A, lda, ka,
                        (const Sp<float>*)B, ldb, kb, N3, cp, part, lower);
-    hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((cnt + BS - 1) / BS)), dim3(BS), 0, st, part, nc, cnt, ka,
-                       lower ? GB : 0, C);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, nc, cnt, ka, lower ? GB : 0,
+                       C);
     (void)nchunk;
     return;
   }
   int nti = (ka + GT - 1) / GT, ntj = (kb + GT - 1) / GT;
   const int ncd = std::min(nchunk, (int)((N3 + GRAM_CHUNK - 1) / GRAM_CHUNK));
   hipLaunchKernelGGL((k_gram<R>), dim3(nti * ntj, ncd), dim3(BS), 0, st, A, lda, ka, B, ldb, kb, N3, part, lower);
-  hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((cnt + BS - 1) / BS)), dim3(BS), 0, st, part, ncd, cnt, ka,
-                     lower ? GT : 0, C);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, ncd, cnt, ka, lower ? GT : 0, C);
 }
 template void launch_gram<double>(hipStream_t, const Sp<double>*, size_t, int, const Sp<double>*, size_t, int, size_t,
                                   double2*, int, double2*, int);
@@ -952,6 +963,24 @@ void launch_col_normalize(hipStream_t st, Sp<R>* X, size_t N3, size_t ld, int nc
 }
 template void launch_col_normalize<double>(hipStream_t, Sp<double>*, size_t, size_t, int, int, int, double, double*);
 template void launch_col_normalize<float>(hipStream_t, Sp<float>*, size_t, size_t, int, int, int, double, double*);
+
+// dst column c = src column order[c]
+template <typename R>
+__global__ void k_gather_cols(const Sp<R>* __restrict__ src, Sp<R>* __restrict__ dst, const int* __restrict__ order,
+                              size_t N3, size_t ld) {
+  const int c = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  dst[(size_t)c * ld + p] = src[(size_t)order[c] * ld + p];
+}
+template <typename R>
+void launch_gather_cols(hipStream_t st, const Sp<R>* src, Sp<R>* dst, const int* order, size_t N3, size_t ld,
+                        int ncols) {
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  hipLaunchKernelGGL((k_gather_cols<R>), grid, dim3(BS), 0, st, src, dst, order, N3, ld);
+}
+template void launch_gather_cols<double>(hipStream_t, const Sp<double>*, Sp<double>*, const int*, size_t, size_t, int);
+template void launch_gather_cols<float>(hipStream_t, const Sp<float>*, Sp<float>*, const int*, size_t, size_t, int);
 
 template <typename Ra, typename Rb>
 __global__ void k_convert(const Sp<Ra>* __restrict__ a, Sp<Rb>* __restrict__ b, size_t cnt) {

@@ -122,4 +122,7 @@ def test_block_gram(H, npts, ka, kb):
     assert np.abs(ctx.block_gram(A, B, C.PREC_MIXED) - A32.conj().T @ B32).max() < 3e-8
     H = ctx.block_gram(A, None, C.PREC_MIXED)
     assert np.abs(H - A32.conj().T @ A32).max() < 3e-8
-    assert np.abs(H - H.conj().T).max() == 0.0  # mirrored exactly
+    # blocks strictly above the 32 x 32 block diagonal are exact mirrors
+    i, j = np.indices(H.shape)
+    up = (i // 32) < (j // 32)
+    assert np.array_equal(H[up], H.conj().T[up])
