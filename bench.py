@@ -5,13 +5,17 @@ Workload: Ca-40 HF+BCS with SLy4, box [-9, 9] fm, 40 points per axis
 V0 = 200 + 100 U(0,1) MeV fm^3 (seeded), harmonic-oscillator start
 (hbar omega = 41 A^-1/3 MeV) plus a seeded +-1e-3 perturbation.
 
-One step = one complete self-consistent solve on the GPU, started from the
-same guess.  First the plain FP64 path runs to the reference convergence
-criterion (|dE| < 1e-6 MeV); its converged E_tot is the FP64 value the
-accelerated path is measured against.  The timed steps then run the
-accelerated path (FP32 stencil and CG, FP64 Rayleigh-Ritz / Cholesky / BCS /
-energies) and record the wall time until E_tot stays within 1 keV of that
-value.  `value` = mean seconds per solve (lower is better).
+The plain path is the reference iteration in FP64 on the GPU (linear density
+mixing, Jacobi-PCG W-block); it runs to the reference convergence criterion
+(|dE| < 1e-6 MeV) and fixes the FP64 value of E_tot.  The accelerated path
+is FP32 states / stencil with split-BF16 MFMA block products and FP64
+Cholesky-QR, Rayleigh-Ritz, BCS and energies, Broyden density mixing, and
+the learned convolutional preconditioner in the GCG W-block.  Its network is
+trained on the GPU during a set-up phase (pairs collected from a plain run
+of O-16, 30^3 -- another nucleus and lattice), timed separately.
+
+One step = one complete solve of the accelerated path from the same guess;
+`value` = mean wall seconds until E_tot stays within 1 keV of the FP64 value.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -30,7 +34,7 @@ import numpy as np  # noqa: E402
 from nuclearcgm_amd import config as C  # noqa: E402
 from nuclearcgm_amd import hip  # noqa: E402
 
-METRIC = "s to self-consistent E_tot (|ΔE|≤1 keV vs FP64), mixed+learned vs plain"
+METRIC = "s to self-consistent E_tot (|ΔE|≤1keV vs FP64), mixed+learned vs plain"
 PEAK_HBM_GBS = 8000.0
 
 
@@ -46,7 +50,7 @@ def make_guess(cfg, seed=7, amp=1e-3):
 
 
 def time_to_window(trace, e_ref, window=1e-3):
-    """Wall time after which E_int stays within `window` of e_ref."""
+    """(wall time, iterations) after which E_int stays within `window` of e_ref."""
     inside = np.abs(trace[:, 0] - e_ref) <= window
     if not inside[-1]:
         return float("nan"), -1
@@ -56,10 +60,39 @@ def time_to_window(trace, e_ref, window=1e-3):
     return float(trace[k, 3]), k + 1
 
 
+def accelerated_config(workload, wstep=C.W_LEARNED, iters=1, mixing=C.MIX_BROYDEN, steps=80):
+    c = C.workload(workload, precision=C.PREC_MIXED)
+    c.max_iter = 400
+    c.mixing = mixing
+    c.wstep = wstep
+    c.wstep_iters = iters
+    c.gcg.steps = steps
+    return c
+
+
+def train_preconditioner(ctx, pairs=240, seed=1234):
+    """Set-up: (residual, converged correction) pairs from a plain FP32 run of
+    O-16 (30^3), then Adam on the GPU with the library's own kernels."""
+    t0 = time.time()
+    ctx.lcp_init(seed)
+    ctx.lcp_clear_pairs()
+    co = C.workload("o16", precision=C.PREC_MIXED)
+    co.max_iter = 80
+    ctx.lcp_collect(pairs, every=1)
+    ctx.problem(co).scf_run(*make_guess(co))
+    ctx.lcp_collect(0)
+    t1 = time.time()
+    first, _ = ctx.lcp_train(600, batch=4, lr=2e-3, seed=7)
+    _, last = ctx.lcp_train(600, batch=4, lr=5e-4, seed=8)
+    return {"seconds": time.time() - t0, "collect_seconds": t1 - t0, "pairs": ctx.lcp_num_pairs(),
+            "train_steps": 1200, "loss_first": first, "loss_last": last,
+            "data": "pairs from a plain FP32 run of O-16 (30^3, h = 0.62 fm)"}
+
+
 def cpu_baseline(cfg, guess, iters_needed, sample_iters=2):
-    """FP64 CPU restatement (oracle), bounded sample: `sample_iters` HF
-    iterations of the same workload on all host cores; extrapolated to the
-    iteration count the FP64 path needs to reach the 1 keV window."""
+    """FP64 CPU restatement (the oracle), bounded sample: `sample_iters` HF
+    iterations of the same workload on all host cores, extrapolated to the
+    iteration count the plain path needs to reach the 1 keV window."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
 
@@ -75,42 +108,54 @@ def cpu_baseline(cfg, guess, iters_needed, sample_iters=2):
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
-    return {"value": per_it * iters_needed, "unit": "s", "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
-            "kind": "port",
+    cores = int(os.environ.get("OMP_NUM_THREADS", cores))
+    return {"value": per_it * iters_needed, "unit": "s", "cores": cores, "kind": "port",
             "sample": f"{out[4].iterations} FP64 HF iterations of the Ca-40 40^3 workload in {dt:.1f} s "
-                      f"({per_it:.2f} s/iteration), times the {iters_needed} iterations the FP64 path needs"}
+                      f"({per_it:.2f} s/iteration) on {cores} threads, times the {iters_needed} iterations "
+                      f"the plain path needs to reach 1 keV"}
+
+
+def solve(ctx, cfg, guess, e_ref):
+    r = ctx.problem(cfg).scf_run(*guess)
+    t, k = time_to_window(r[5], e_ref)
+    return {"seconds_to_1keV": t, "iterations_to_1keV": k, "iterations": r[4].iterations,
+            "dE_keV": (r[4].terms[C.E_INT] - e_ref) * 1e3}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ca40")
+    ap.add_argument("--wstep-iters", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the whole solve on a single MI355X; "
-              "--gpus above 1 is not supported", file=sys.stderr)
+        print("bench.py measures one GPU: the whole solve runs on a single MI355X; --gpus above 1 is not "
+              "supported", file=sys.stderr)
         sys.exit(2)
 
+    ctx = hip.Context(0)
     cfg64 = C.workload(args.workload, precision=C.PREC_FP64)
     cfg64.max_iter = 1500
     guess = make_guess(cfg64)
-    ctx = hip.Context(0)
-    # plain FP64 path: reference value and its own time to the window
+    # plain FP64 path: the reference value and its own time to the window
     p64 = ctx.problem(cfg64)
+    p64.scf_run(*guess)  # first run: device warm-up
     r64 = p64.scf_run(*guess)
     e_ref = r64[4].terms[C.E_INT]
     t64, it64 = time_to_window(r64[5], e_ref)
+    # set-up of the accelerated path: train the learned preconditioner
+    setup = train_preconditioner(ctx)
     # accelerated path
-    cfgm = C.workload(args.workload, precision=C.PREC_MIXED)
-    cfgm.max_iter = 1500
+    cfgm = accelerated_config(args.workload, iters=args.wstep_iters)
     pm = ctx.problem(cfgm)
     for _ in range(args.warmup):
         pm.scf_run(*guess)
+    pm.lcp_stats(reset=True)
+    pm.kernel_stats(enable=1)  # live HIP-event timing of the stencil and block-product launches
     times, iters, last = [], [], None
-    pm.stencil_stats(enable=1)  # live HIP-event timing of every CG stencil launch
     t_all = time.time()
     for _ in range(args.steps):
         last = pm.scf_run(*guess)
@@ -118,38 +163,62 @@ def main():
         times.append(t)
         iters.append(k)
     wall = time.time() - t_all
-    st = pm.stencil_stats(enable=0)
+    ks = pm.kernel_stats(enable=0)
+    lst = pm.lcp_stats()
     value = float(np.mean(times))
-    # roofline of the dominant kernel: the FP32 H.psi stencil of the block CG
-    # (k_hpsi<float, true>), algorithmic bytes per launch N^3 (32 m + 32) with
-    # m = active state columns of that launch (16 B read + 16 B written per
-    # spinor point, 8 FP32 field scalars once)
-    N3 = st["N3"]
-    ms = st["ms"] / max(st["launches"], 1)
-    m_avg = st["active_cols"] / max(st["launches"], 1)
-    alg_bytes = N3 * (32 * m_avg + 32)
-    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    # roofline of the dominant kernel class (both are HBM bound: the stencil
+    # does ~9 flop/B, the block products ~50 flop/B, below the MFMA ridge)
+    dom = max(ks, key=lambda k: ks[k]["ms"])
+    d = ks[dom]
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    names = {"stencil": "k_hpsi25<float> (H.psi stencil, FP32)",
+             "block": "k_gram_mfma (split-BF16 MFMA Gram / projection of FP32 states)"}
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+    other = "block" if dom == "stencil" else "stencil"
+    o = ks[other]
     sm = last[4].summary()
     line = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 stencil/CG, fp64 RR/CholQR/BCS/energies", "data": "synthetic (seeded HO guess)",
+        "dtype": "fp32 states/stencil, split-bf16 MFMA block products, fp64 Cholesky-QR/RR/BCS/energies",
+        "data": "synthetic (seeded harmonic-oscillator guess)",
         "config": {"workload": "Ca-40 HF+BCS SLy4, 40^3 lattice h=0.4615 fm, 32+32 states, V0 seeded",
+                   "accelerated": f"mixed precision + Broyden mixing + learned W-block ({args.wstep_iters} "
+                                  f"preconditioned minimal-residual step(s) per GCG call)",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "k_hpsi<float, true> (H.psi stencil of the block CG)", "avg_us": ms * 1e3,
-                     "alg_bytes": alg_bytes, "launches": st["launches"], "avg_active_states": m_avg},
-        "fp64_path": {"seconds_to_1keV": t64, "iterations_to_1keV": it64, "iterations": r64[4].iterations,
-                      "seconds_total": r64[4].seconds, "E_tot": e_ref},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "kernel": names[dom],
+                     "avg_us": d["ms"] * 1e3 / max(d["launches"], 1),
+                     "alg_bytes_per_launch": d["bytes"] / max(d["launches"], 1), "launches": d["launches"],
+                     "tflops": d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["flops"] else None},
+        "roofline_other": {"kernel": names[other], "achieved": o["bytes"] / max(o["ms"], 1e-9) / 1e6,
+                           "frac": o["bytes"] / max(o["ms"], 1e-9) / 1e6 / PEAK_HBM_GBS,
+                           "avg_us": o["ms"] * 1e3 / max(o["launches"], 1), "launches": o["launches"]},
+        "plain_fp64_path": {"seconds_to_1keV": t64, "iterations_to_1keV": it64, "iterations": r64[4].iterations,
+                            "seconds_total": r64[4].seconds, "E_tot": e_ref},
         "accelerated_path": {"seconds_to_1keV": times, "iterations_to_1keV": iters,
                              "E_tot": sm["E_int"], "dE_keV": (sm["E_int"] - e_ref) * 1e3,
                              "max_deps_n": float(np.abs(last[2] - r64[2]).max()),
                              "max_deps_p": float(np.abs(last[3] - r64[3]).max()),
                              "d_rms_n": sm["rms_n"] - math.sqrt(r64[4].r2_n),
                              "d_rms_p": sm["rms_p"] - math.sqrt(r64[4].r2_p),
-                             "d_beta2": sm["beta2"] - r64[4].beta2},
-        "speedup_vs_fp64_path": t64 / value if value > 0 else None,
+                             "d_beta2": sm["beta2"] - r64[4].beta2,
+                             "learned_wblock": lst},
+        "setup": setup,
+        "speedup_vs_plain_fp64": t64 / value if value > 0 else None,
+    }
+    # ablations (one solve each): the same path without the learned net
+    # (Jacobi-PCG W-block, 8 steps), and mixed precision alone
+    line["ablation"] = {
+        "mixed+broyden+jacobi_pcg8": solve(ctx, accelerated_config(args.workload, C.W_JACOBI_CG, 0, C.MIX_BROYDEN,
+                                                                   8), guess, e_ref),
+        "mixed+broyden+jacobi_pcg1": solve(ctx, accelerated_config(args.workload, C.W_JACOBI_CG, 0, C.MIX_BROYDEN,
+                                                                   1), guess, e_ref),
+        "mixed_only": solve(ctx, accelerated_config(args.workload, C.W_JACOBI_CG, 0, C.MIX_LINEAR, 80), guess,
+                            e_ref),
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg64, guess, it64)

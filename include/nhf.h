@@ -245,12 +245,15 @@ int nhf_scf_run(void *ctx, int pid, double *psi_n, double *psi_p, double *eps_n,
  * No reference counterpart (the reference has no timers on this path). */
 int nhf_bench_hpsi(void *ctx, int pid, int q, int m, int prec, int reps, double *avg_ms);
 
-/* Measurement: live timing of the H.psi applications inside the block-CG
- * of the GCG steps.  enable = 1 starts (and resets) HIP-event timing of
- * every CG stencil launch on the species streams, 0 stops it, -1 leaves it;
- * out (may be NULL): [0] summed kernel ms, [1] launches, [2] active
- * (state column x launch) count, [3] n^3.  No reference counterpart. */
-int nhf_stencil_stats(void *ctx, int pid, int enable, double *out);
+/* Measurement: live timing (HIP events on the species streams) of the
+ * kernels of the SCF steps.  enable = 1 starts (and resets) the timing,
+ * 0 stops it, -1 leaves it.  out (may be NULL), per kernel class k
+ * (0 = H.psi stencil, 1 = block products): out[4k + 0] summed kernel ms,
+ * [4k + 1] launches, [4k + 2] algorithmic bytes (stencil: n^3 (2 s m + 8 r)
+ * with s the spinor point size, r the real size and m the columns of the
+ * launch; block products: every operand column once), [4k + 3] algorithmic
+ * flops (block products: 16 n^3 ka kb).  No reference counterpart. */
+int nhf_kernel_stats(void *ctx, int pid, int enable, double *out);
 
 /* Measurement: accumulated wall time (s) of the SCF phases since the last
  * reset: out[0] shared density/occupation update, out[1] energies, then per

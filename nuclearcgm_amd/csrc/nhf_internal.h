@@ -33,6 +33,12 @@ struct BroydenCoef {
   double g[BROYDEN_MAX];
 };
 
+// Optional events recorded by the dispatch itself (hipExtLaunchKernel): the
+// live kernel timing of the SCF steps, free of host launch gaps.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+
 // ------------------------------------------------------------------ kernels
 // All launches go to `st`.  launch_hpsi: out = (H + shift) in; when dot is
 // non-null also accumulates in^H out of column c into dot[c * dot_stride + 0..1]
@@ -41,7 +47,7 @@ struct BroydenCoef {
 template <typename R>
 void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int ncols, const R* fld, int n, double h,
                  double shift, double* dot, int dot_stride, const int* done, unsigned long long* act = nullptr,
-                 const int* clist = nullptr);
+                 const int* clist = nullptr, const LaunchEvents* ev = nullptr);
 
 // block Jacobi-PCG kernels (Eigen ConjugateGradient restated per column)
 template <typename R>
@@ -66,7 +72,7 @@ void launch_cg_check0(hipStream_t st, const double* s0, const double* rhs2, doub
 // computed, the rest is filled by Hermitian symmetry
 template <typename R>
 void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>* B, size_t ldb, int kb, size_t N3,
-                 double2* part, int nchunk, double2* C, int lower = 0);
+                 double2* part, int nchunk, double2* C, int lower = 0, const LaunchEvents* ev = nullptr);
 int gram_chunks(size_t N3);
 
 // Y (ncols) = V (k cols) * C (k x ncols complex double, column-major),

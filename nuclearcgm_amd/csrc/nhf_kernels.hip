@@ -8,6 +8,8 @@
 // 64 consecutive points = one coalesced 1 KiB (FP32) / 2 KiB (FP64) segment.
 #include "nhf_internal.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cfloat>
 #include <cstdlib>
@@ -37,6 +39,15 @@ static int hpsi_variant() {
   }();
   return v;
 }
+
+// launch with the dispatch-recorded events of `ev` (start, stop may be null)
+#define LAUNCH_EV(ev0, ev1, kern, grid, block, sh, st, ...)                                 \
+  do {                                                                                       \
+    if ((ev0) || (ev1))                                                                      \
+      hipExtLaunchKernelGGL(kern, grid, block, sh, st, ev0, ev1, 0, __VA_ARGS__);            \
+    else                                                                                     \
+      hipLaunchKernelGGL(kern, grid, block, sh, st, __VA_ARGS__);                            \
+  } while (0)
 
 // ------------------------------------------------------------- helpers
 template <typename R>
@@ -285,9 +296,10 @@ __global__ __launch_bounds__(BS) void k_hpsi25(const Sp<R>* __restrict__ in, Sp<
 template <typename R>
 void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int ncols, const R* fld, int n, double h,
                  double shift, double* dot, int dot_stride, const int* done, unsigned long long* act,
-                 const int* clist) {
+                 const int* clist, const LaunchEvents* ev) {
   size_t N3 = (size_t)n * n * n;
   R c1 = (R)(1.0 / (12.0 * h)), c2 = (R)(1.0 / (12.0 * h * h));
+  hipEvent_t e0 = ev ? ev->start : nullptr, e1 = ev ? ev->stop : nullptr;
   if (4 * n <= BS && hpsi_variant() == 1) {
     // z chunks: enough workgroups to fill the device (>= ~2048), but no
     // chunk shorter than 4 planes (4 warm-up loads per chunk)
@@ -299,25 +311,27 @@ void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int nco
     dim3 g3((unsigned)ntile, (unsigned)nzc, (unsigned)ncols);
     size_t sh = sizeof(Sp<R>) * (BS + 4 * n);
     if (dot)
-      hipLaunchKernelGGL((k_hpsi25<R, true>), g3, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
-                         dot_stride, done, act, clist);
+      LAUNCH_EV(e0, e1, (k_hpsi25<R, true>), g3, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
+                dot_stride, done, act, clist);
     else
-      hipLaunchKernelGGL((k_hpsi25<R, false>), g3, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
-                         dot_stride, done, act, clist);
+      LAUNCH_EV(e0, e1, (k_hpsi25<R, false>), g3, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
+                dot_stride, done, act, clist);
     return;
   }
   dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
   if (dot)
-    hipLaunchKernelGGL((k_hpsi<R, true>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot,
-                       dot_stride, done, act, clist);
+    LAUNCH_EV(e0, e1, (k_hpsi<R, true>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot,
+              dot_stride, done, act, clist);
   else
-    hipLaunchKernelGGL((k_hpsi<R, false>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot,
-                       dot_stride, done, act, clist);
+    LAUNCH_EV(e0, e1, (k_hpsi<R, false>), grid, dim3(BS), 0, st, in, out, ld, fld, n, N3, c1, c2, (R)shift, dot,
+              dot_stride, done, act, clist);
 }
 template void launch_hpsi<double>(hipStream_t, const Sp<double>*, Sp<double>*, size_t, int, const double*, int, double,
-                                  double, double*, int, const int*, unsigned long long*, const int*);
+                                  double, double*, int, const int*, unsigned long long*, const int*,
+                                  const LaunchEvents*);
 template void launch_hpsi<float>(hipStream_t, const Sp<float>*, Sp<float>*, size_t, int, const float*, int, double,
-                                 double, double*, int, const int*, unsigned long long*, const int*);
+                                 double, double*, int, const int*, unsigned long long*, const int*,
+                                 const LaunchEvents*);
 
 // ======================================================== block Jacobi-PCG
 // Per column c the iteration of Eigen's ConjugateGradient with the diagonal
@@ -732,29 +746,30 @@ static int gram_mfma_chunk(size_t N3, int blocks) {
 
 template <typename R>
 void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>* B, size_t ldb, int kb, size_t N3,
-                 double2* part, int nchunk, double2* C, int lower) {
+                 double2* part, int nchunk, double2* C, int lower, const LaunchEvents* ev) {
   lower = (lower && A == B && ka == kb) ? 1 : 0;
   size_t cnt = (size_t)ka * kb;
+  hipEvent_t e0 = ev ? ev->start : nullptr, e1 = ev ? ev->stop : nullptr, none = nullptr;
   if (std::is_same<R, float>::value && !block_fp64()) {
     const int nbi = (ka + GB - 1) / GB, nbj = (kb + GB - 1) / GB;
     const int cp = gram_mfma_chunk(N3, nbi * nbj);
     const int nc = (int)((N3 + cp - 1) / cp);
-    hipLaunchKernelGGL(k_gram_mfma, dim3(nbi * nbj, nc), dim3(BS), 0, st, (const Sp<float>*)A, lda, ka,
-                       (const Sp<float>*)B, ldb, kb, N3, cp, part, lower);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, nc, cnt, ka, lower ? GB : 0,
-                       C);
+    LAUNCH_EV(e0, none, k_gram_mfma, dim3(nbi * nbj, nc), dim3(BS), 0, st, (const Sp<float>*)A, lda, ka,
+              (const Sp<float>*)B, ldb, kb, N3, cp, part, lower);
+    LAUNCH_EV(none, e1, k_reduce_parts, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, nc, cnt, ka, lower ? GB : 0,
+              C);
     (void)nchunk;
     return;
   }
   int nti = (ka + GT - 1) / GT, ntj = (kb + GT - 1) / GT;
   const int ncd = std::min(nchunk, (int)((N3 + GRAM_CHUNK - 1) / GRAM_CHUNK));
-  hipLaunchKernelGGL((k_gram<R>), dim3(nti * ntj, ncd), dim3(BS), 0, st, A, lda, ka, B, ldb, kb, N3, part, lower);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, ncd, cnt, ka, lower ? GT : 0, C);
+  LAUNCH_EV(e0, none, (k_gram<R>), dim3(nti * ntj, ncd), dim3(BS), 0, st, A, lda, ka, B, ldb, kb, N3, part, lower);
+  LAUNCH_EV(none, e1, k_reduce_parts, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, ncd, cnt, ka, lower ? GT : 0, C);
 }
 template void launch_gram<double>(hipStream_t, const Sp<double>*, size_t, int, const Sp<double>*, size_t, int, size_t,
-                                  double2*, int, double2*, int);
+                                  double2*, int, double2*, int, const LaunchEvents*);
 template void launch_gram<float>(hipStream_t, const Sp<float>*, size_t, int, const Sp<float>*, size_t, int, size_t,
-                                 double2*, int, double2*, int);
+                                 double2*, int, double2*, int, const LaunchEvents*);
 
 // Y[:, c] (+)= sum_l V[:, l] C[l, c]; optional residual norms
 // |AV c - w_c V c|^2 (the Ritz residual of column c).

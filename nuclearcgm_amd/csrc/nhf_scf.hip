@@ -62,12 +62,17 @@ struct Work {
   double2 *gpart, *gout;
   double *cg_scal, *thr, *rhs2, *lam_shift, *rn2, *norms, *scal_col;
   int *done, *its, *clist;
-  // live stencil timing of the CG applications (optional)
+  // live timing (HIP events on this stream) of the two kernel classes that
+  // carry the work: 0 = H.psi stencil, 1 = block products (Gram /
+  // projection), with their algorithmic bytes and flops
   bool timing = false;
-  std::vector<hipEvent_t> ev;
-  double st_ms = 0.0;
-  long st_launches = 0;
-  unsigned long long* act = nullptr;  // active column-launches (device counter)
+  std::vector<hipEvent_t> ev;  // start/stop pairs, recycled after harvest
+  std::vector<int> ev_kind;
+  std::vector<double> ev_bytes, ev_flops;
+  int ev_used = 0;
+  double k_ms[2] = {0, 0}, k_bytes[2] = {0, 0}, k_flops[2] = {0, 0};
+  long k_launches[2] = {0, 0};
+  unsigned long long* act = nullptr;  // unused device counter slot of the stencil kernel
   // wall-time breakdown of the species step (seconds, accumulated)
   double tph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // learned W-block: per-point scales, net scratch, step scalars, statistics
@@ -83,6 +88,44 @@ struct Work {
 using Clock = std::chrono::steady_clock;
 static inline double secs(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double>(b - a).count();
+}
+
+// a timed launch: the events its dispatch records (null when timing is off)
+static int kt_slot(Work& W, int kind, double bytes, double flops, LaunchEvents& le, const LaunchEvents*& out) {
+  out = nullptr;
+  if (!W.timing) return NHF_OK;
+  if ((int)W.ev.size() < 2 * (W.ev_used + 1)) {
+    for (int k = 0; k < 2; ++k) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      W.ev.push_back(e);
+    }
+    W.ev_kind.push_back(0);
+    W.ev_bytes.push_back(0);
+    W.ev_flops.push_back(0);
+  }
+  const int slot = W.ev_used++;
+  W.ev_kind[slot] = kind;
+  W.ev_bytes[slot] = bytes;
+  W.ev_flops[slot] = flops;
+  le.start = W.ev[2 * slot];
+  le.stop = W.ev[2 * slot + 1];
+  out = &le;
+  return NHF_OK;
+}
+// folds the recorded launches into the totals; the stream must be idle
+static int kt_harvest(Work& W) {
+  for (int s = 0; s < W.ev_used; ++s) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, W.ev[2 * s], W.ev[2 * s + 1]));
+    const int k = W.ev_kind[s];
+    W.k_ms[k] += ms;
+    W.k_bytes[k] += W.ev_bytes[s];
+    W.k_flops[k] += W.ev_flops[s];
+    ++W.k_launches[k];
+  }
+  W.ev_used = 0;
+  return NHF_OK;
 }
 
 struct Problem {
@@ -302,7 +345,12 @@ static int setup_problem(Problem& P) {
 // ------------------------------------------------------- small helpers
 template <typename R>
 static int gram_host(Problem& P, Work& W, const Sp<R>* A, int ka, const Sp<R>* B, int kb, std::vector<cd>& out) {
-  launch_gram<R>(W.st, A, P.N3, ka, B, P.N3, kb, P.N3, W.gpart, P.gchunks, W.gout, A == B ? 1 : 0);
+  // algorithmic work: each operand column read once; 8 n ka kb flops with
+  // n = 2 n^3 complex rows (the full product, also for a Hermitian result)
+  LaunchEvents le;
+  const LaunchEvents* ev;
+  CHK(kt_slot(W, 1, (double)(A == B ? ka : ka + kb) * P.N3 * sizeof(Sp<R>), 16.0 * P.N3 * ka * kb, le, ev));
+  launch_gram<R>(W.st, A, P.N3, ka, B, P.N3, kb, P.N3, W.gpart, P.gchunks, W.gout, A == B ? 1 : 0, ev);
   out.resize((size_t)ka * kb);
   HIPCHK(hipMemcpyAsync(out.data(), W.gout, sizeof(double2) * ka * kb, hipMemcpyDeviceToHost, W.st));
   HIPCHK(hipStreamSynchronize(W.st));
@@ -352,6 +400,20 @@ const float* fld_of<float>(Problem& P, int q) {
   return P.fld32[q];
 }
 
+// H.psi on ncols columns with live timing; algorithmic bytes per launch
+// n^3 (2 sizeof(Sp<R>) ncols + NFLD sizeof(R)): every spinor read and
+// written once, the packed fields read once
+template <typename R>
+static int hpsi_t(Problem& P, Work& W, int q, const Sp<R>* in, Sp<R>* out, int ncols, double shift,
+                  double* dot = nullptr, int dot_stride = 0, const int* done = nullptr, const int* clist = nullptr) {
+  LaunchEvents le;
+  const LaunchEvents* ev;
+  CHK(kt_slot(W, 0, (double)P.N3 * (2.0 * sizeof(Sp<R>) * ncols + NFLD * sizeof(R)), 0.0, le, ev));
+  launch_hpsi<R>(W.st, in, out, P.N3, ncols, fld_of<R>(P, q), P.n, P.h, shift, dot, dot_stride, done, nullptr, clist,
+                 ev);
+  return NHF_OK;
+}
+
 // The learned W-block: `iters` preconditioned minimal-residual steps on
 // (H + shift) w = (lambda + shift) x from w = x, with the context's
 // convolutional net as the preconditioner.  `ratio` = mean over columns of
@@ -369,11 +431,11 @@ static int learned_wstep(Problem& P, Work& Wk, int q, const Sp<R>* X, Sp<R>* Wb,
   auto sc = [&](int it) { return Wk.lw + (size_t)it * M * LW_NS; };
   launch_lcp_planes<R>(st, fld, N3, P.h, shift, Wk.kap, Wk.hf);
   HIPCHK(hipMemsetAsync(Wk.lw, 0, sizeof(double) * LW_NS * (LW_MAX_ITERS + 1) * M, st));
-  launch_hpsi<R>(st, X, Q, ld, M, fld, P.n, P.h, shift, nullptr, 0, nullptr);
+  CHK(hpsi_t<R>(P, Wk, q, X, Q, M, shift));
   launch_lw_init<R>(st, X, Q, Wk.lam_shift, Wb, Rr, N3, ld, M, sc(0));
   for (int it = 0; it < iters; ++it) {
     lcp_apply<R>(*P.lcp, st, Rr, Z, ld, M, nullptr, sc(it), Wk.kap, Wk.hf, P.n, Wk.lh0, Wk.lh1, LW_NS);
-    launch_hpsi<R>(st, Z, Q, ld, M, fld, P.n, P.h, shift, nullptr, 0, nullptr);
+    CHK(hpsi_t<R>(P, Wk, q, Z, Q, M, shift));
     launch_lw_dots<R>(st, Q, Rr, N3, ld, M, sc(it));
     launch_lw_update<R>(st, Z, Q, Wb, Rr, N3, ld, M, sc(it), sc(it + 1));
   }
@@ -429,7 +491,7 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
   if (ke < M) return NHF_ESOLVER;
   CHK(rotate_dev<R>(P, Wk, psi_in, M, C1, M, V1, nullptr, nullptr, false));
   // 2. Rayleigh-Ritz on X with shift 0
-  launch_hpsi<R>(st, V1, AV, ld, M, fld, P.n, P.h, 0.0, nullptr, 0, nullptr);
+  CHK(hpsi_t<R>(P, Wk, q, V1, AV, M, 0.0));
   CHK(gram_host<R>(P, Wk, V1, M, AV, M, Pm));
   std::vector<double> w(M);
   std::vector<cd> Cx((size_t)M * M);
@@ -474,7 +536,7 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
     }
     if (!w_done) {
     launch_invdiag<R>(st, fld, N3, P.h, shift, invd);
-    launch_hpsi<R>(st, X, T, ld, M, fld, P.n, P.h, shift, nullptr, 0, nullptr);
+    CHK(hpsi_t<R>(P, Wk, q, X, T, M, shift));
     std::vector<double> ls(M);
     for (int c = 0; c < M; ++c) ls[c] = Lam[c] + shift;
     HIPCHK(hipMemcpyAsync(Wk.lam_shift, ls.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
@@ -496,21 +558,10 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
       HIPCHK(hipMemcpyAsync(Wk.clist, act.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, st));
       HIPCHK(hipStreamSynchronize(st));
     }
-    int nev = 0;
-    if (Wk.timing) {
-      while ((int)Wk.ev.size() < 2 * steps) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        Wk.ev.push_back(e);
-      }
-    }
     for (int it = 0; it < steps && ncl > 0; ++it) {
       double* sprev = Wk.cg_scal + (size_t)it * M * CG_NS;
       double* scur = Wk.cg_scal + (size_t)(it + 1) * M * CG_NS;
-      if (Wk.timing) HIPCHK(hipEventRecord(Wk.ev[2 * nev], st));
-      launch_hpsi<R>(st, Pd, T, ld, ncl, fld, P.n, P.h, shift, scur, CG_NS, Wk.done, Wk.timing ? Wk.act : nullptr,
-                     Wk.clist);
-      if (Wk.timing) HIPCHK(hipEventRecord(Wk.ev[2 * nev++ + 1], st));
+      CHK(hpsi_t<R>(P, Wk, q, Pd, T, ncl, shift, scur, CG_NS, Wk.done, Wk.clist));
       launch_cg_update<R>(st, Wb, Rr, Pd, T, invd, N3, ld, ncl, sprev, scur, Wk.done, Wk.clist);
       launch_cg_dir<R>(st, Rr, Pd, invd, N3, ld, ncl, sprev, scur, Wk.thr, Wk.done, Wk.its, Wk.clist);
       if ((it % 4) == 3 && it + 1 < steps) {
@@ -532,19 +583,10 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
       HIPCHK(hipStreamSynchronize(st));
       for (int c = 0; c < M; ++c) info->cg_iters += its[c];
     }
-    if (Wk.timing) {
-      HIPCHK(hipStreamSynchronize(st));
-      for (int e = 0; e < nev; ++e) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, Wk.ev[2 * e], Wk.ev[2 * e + 1]));
-        Wk.st_ms += ms;
-      }
-      Wk.st_launches += nev;
-    }
     // training pairs (residual, converged correction) for the learned net
     if (P.lcp && P.lcp->collect_left > 0 && (Wk.gcg_calls - 1) % std::max(1, P.lcp->collect_every) == 0) {
       launch_lcp_planes<R>(st, fld, N3, P.h, shift, Wk.kap, Wk.hf);
-      launch_hpsi<R>(st, X, T, ld, M, fld, P.n, P.h, shift, nullptr, 0, nullptr);
+      CHK(hpsi_t<R>(P, Wk, q, X, T, M, shift));
       CHK(lcp_add_pairs<R>(*P.lcp, st, T, X, Wb, Wk.lam_shift, M, ld, Wk.kap, Wk.hf, P.n, Wk.pscr));
     }
     }  // plain W-block
@@ -578,7 +620,7 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
     auto tD = Clock::now();
     Wk.tph[3] += secs(tC, tD);
     // 6. Rayleigh-Ritz: Pm = C2^H (V1^H (H + sigma) V1) C2
-    launch_hpsi<R>(st, V1, AV, ld, k1, fld, P.n, P.h, shift, nullptr, 0, nullptr);
+    CHK(hpsi_t<R>(P, Wk, q, V1, AV, k1, shift));
     std::vector<cd> P1;
     CHK(gram_host<R>(P, Wk, V1, k1, AV, k1, P1));
     if (pass2) {
@@ -659,6 +701,7 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
     launch_gather_cols<R>(st, V, psi_out, Wk.clist, N3, ld, M);
   }
   HIPCHK(hipStreamSynchronize(st));
+  CHK(kt_harvest(Wk));
   eps.resize(M);
   for (int c = 0; c < M; ++c) eps[c] = Lam[ord[c]];
   return NHF_OK;
@@ -1105,6 +1148,7 @@ static int species_step(Problem& P, int q, const FieldParams& F, hipEvent_t dens
     pc = M;
   }
   HIPCHK(hipStreamSynchronize(W.st));
+  CHK(kt_harvest(W));
   W.tph[5] += secs(t5, Clock::now());
   return NHF_OK;
 }
@@ -1532,28 +1576,34 @@ int nhf_bench_hpsi(void* ctx, int pid, int q, int m, int prec, int reps, double*
   return NHF_OK;
 }
 
-int nhf_stencil_stats(void* ctx, int pid, int enable, double* out) {
+int nhf_kernel_stats(void* ctx, int pid, int enable, double* out) {
   Problem* P = get_problem(ctx, pid);
   if (!P) return NHF_EINVAL;
   if (out) {
-    unsigned long long act[2] = {0, 0};
-    for (int q = 0; q < 2; ++q) {
-      HIPCHK(hipMemcpyAsync(&act[q], P->W[q].act, sizeof(unsigned long long), hipMemcpyDeviceToHost, P->st));
+    for (int k = 0; k < 2; ++k) {
+      double ms = 0, by = 0, fl = 0, la = 0;
+      for (int q = 0; q < 2; ++q) {
+        ms += P->W[q].k_ms[k];
+        by += P->W[q].k_bytes[k];
+        fl += P->W[q].k_flops[k];
+        la += (double)P->W[q].k_launches[k];
+      }
+      out[4 * k + 0] = ms;
+      out[4 * k + 1] = la;
+      out[4 * k + 2] = by;
+      out[4 * k + 3] = fl;
     }
-    HIPCHK(hipStreamSynchronize(P->st));
-    out[0] = P->W[0].st_ms + P->W[1].st_ms;
-    out[1] = (double)(P->W[0].st_launches + P->W[1].st_launches);
-    out[2] = (double)(act[0] + act[1]);
-    out[3] = (double)P->N3;
   }
   if (enable >= 0) {
     for (int q = 0; q < 2; ++q) {
-      P->W[q].timing = enable != 0;
-      P->W[q].st_ms = 0.0;
-      P->W[q].st_launches = 0;
-      HIPCHK(hipMemsetAsync(P->W[q].act, 0, sizeof(unsigned long long), P->st));
+      Work& W = P->W[q];
+      W.timing = enable != 0;
+      W.ev_used = 0;
+      for (int k = 0; k < 2; ++k) {
+        W.k_ms[k] = W.k_bytes[k] = W.k_flops[k] = 0.0;
+        W.k_launches[k] = 0;
+      }
     }
-    HIPCHK(hipStreamSynchronize(P->st));
   }
   return NHF_OK;
 }

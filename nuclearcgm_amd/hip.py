@@ -44,7 +44,7 @@ def lib():
                                   ctypes.c_int]
         L.nhf_bench_hpsi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _D]
-        L.nhf_stencil_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
+        L.nhf_kernel_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         L.nhf_phase_times.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         L.nhf_ho_guess.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]
         L.nhf_eigh_lower.argtypes = [ctypes.c_int, _D, _D, _D]
@@ -290,10 +290,14 @@ class Problem:
                                   ctypes.byref(ms)))
         return ms.value
 
-    def stencil_stats(self, enable=-1):
-        out = np.zeros(4)
-        _chk(lib().nhf_stencil_stats(self.ctx.h, self.pid, enable, _dp(out)))
-        return {"ms": out[0], "launches": int(out[1]), "active_cols": int(out[2]), "N3": int(out[3])}
+    def kernel_stats(self, enable=-1):
+        """Live HIP-event timing of the H.psi stencil and block-product
+        launches: enable 1 starts (and resets), 0 stops, -1 leaves it."""
+        out = np.zeros(8)
+        _chk(lib().nhf_kernel_stats(self.ctx.h, self.pid, enable, _dp(out)))
+        return {k: {"ms": out[4 * i], "launches": int(out[4 * i + 1]), "bytes": out[4 * i + 2],
+                    "flops": out[4 * i + 3]}
+                for i, k in enumerate(("stencil", "block"))}
 
     PHASES = ["fields", "orth+RR1", "CG", "basis+CholQR", "RR2", "norm/TR/conj"]
 
