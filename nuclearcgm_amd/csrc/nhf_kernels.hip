@@ -30,6 +30,24 @@ static bool block_fp64() {
   return v;
 }
 
+// 2.5D stencil launch shape: target workgroup count (environment
+// NHF_HPSI_WG, default 2048) and the XCD-aware block order (NHF_HPSI_XCD,
+// default on)
+static int hpsi_wg_target() {
+  static int v = [] {
+    const char* e = getenv("NHF_HPSI_WG");
+    return e ? std::max(1, atoi(e)) : 2048;
+  }();
+  return v;
+}
+static int hpsi_xcd() {
+  static int v = [] {
+    const char* e = getenv("NHF_HPSI_XCD");
+    return e ? (atoi(e) != 0 ? 1 : 0) : 1;
+  }();
+  return v;
+}
+
 // stencil variant: 1 = 2.5D LDS/register kernel (default), 0 = direct loads
 // (environment NHF_HPSI=direct)
 static int hpsi_variant() {
@@ -194,20 +212,32 @@ __global__ __launch_bounds__(BS) void k_hpsi25(const Sp<R>* __restrict__ in, Sp<
                                                const R* __restrict__ fld, int n, R c1, R c2, R shift, int zc,
                                                double* __restrict__ dot, int dot_stride,
                                                const int* __restrict__ done, unsigned long long* __restrict__ act,
-                                               const int* __restrict__ clist) {
+                                               const int* __restrict__ clist, int ntile, int nzc, int ncols,
+                                               int xcd) {
   extern __shared__ char smem[];
   Sp<R>* tile = reinterpret_cast<Sp<R>*>(smem);  // BS + 4n entries
-  const int c = clist ? clist[blockIdx.z] : (int)blockIdx.z;
+  // logical block L = (column, z chunk, plane tile), tile fastest.  With
+  // xcd != 0 the workgroups that the dispatcher sends to one XCD (ids
+  // w = xcd_index (mod 8)) get one contiguous range of L, so that tiles
+  // sharing halo rows and chunks sharing halo planes meet in one L2.
+  const int total = ntile * nzc * ncols;
+  int L = blockIdx.x;
+  if (xcd) {
+    const int per = (total + 7) / 8;
+    L = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (L >= total) return;  // uniform over the workgroup
+  }
+  const int bt = L % ntile, bz = (L / ntile) % nzc, bc = L / (ntile * nzc);
+  const int c = clist ? clist[bc] : bc;
   if (done && done[c]) return;  // uniform over the workgroup
-  if (act && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) atomicAdd(act, 1ull);
+  if (act && bt == 0 && bz == 0 && threadIdx.x == 0) atomicAdd(act, 1ull);
   const int nn = n * n;
   const int tid = threadIdx.x;
-  const int q0 = blockIdx.x * BS;
+  const int q0 = bt * BS;
   const int q = q0 + tid;
   const bool own = q < nn;
-  const int y = own ? q / n : 0;
-  const int x = own ? q - y * n : 0;
-  const int z0 = blockIdx.y * zc;
+  const int x = own ? q % n : 0;
+  const int z0 = bz * zc;
   const int z1 = min(n, z0 + zc);
   const size_t N3 = (size_t)nn * n;
   const Sp<R>* __restrict__ src = in + (size_t)c * ld;
@@ -229,10 +259,20 @@ __global__ __launch_bounds__(BS) void k_hpsi25(const Sp<R>* __restrict__ in, Sp<
     if (hq < 0 || hq >= nn) hq = -1;
   }
   double acc[2] = {0.0, 0.0};
+  // software pipeline: the loads of step z+1 (plane z+3, the halo and the
+  // fields of plane z+1) are issued one step ahead, so their latency hides
+  // behind the current step
+  auto ldh = [&](int z) -> Sp<R> { return (hq >= 0 && z < n) ? src[(size_t)z * nn + hq] : Z; };
+  Sp<R> zp2 = ldz(z0 + 2), hcur = ldh(z0);
+  R F[NFLD];
+  if (own) load_fields(fld, (size_t)z0 * nn + q, F);
   for (int z = z0; z < z1; ++z) {
-    const Sp<R> zp2 = ldz(z + 2);
+    const Sp<R> zp3 = ldz(z + 3);
+    const Sp<R> hnext = (z + 1 < z1) ? ldh(z + 1) : Z;
+    R Fn[NFLD];
+    if (own && z + 1 < z1) load_fields(fld, (size_t)(z + 1) * nn + q, Fn);
     tile[2 * n + tid] = z00;
-    if (hslot >= 0) tile[hslot] = hq >= 0 ? src[(size_t)z * nn + hq] : Z;
+    if (hslot >= 0) tile[hslot] = hcur;
     __syncthreads();
     if (own) {
       const int ct = 2 * n + tid;
@@ -240,10 +280,12 @@ __global__ __launch_bounds__(BS) void k_hpsi25(const Sp<R>* __restrict__ in, Sp<
       const Sp<R> xm2 = x >= 2 ? tile[ct - 2] : Z;
       const Sp<R> xp1 = x + 1 < n ? tile[ct + 1] : Z;
       const Sp<R> xp2 = x + 2 < n ? tile[ct + 2] : Z;
-      const Sp<R> ym1 = y >= 1 ? tile[ct - n] : Z;
-      const Sp<R> ym2 = y >= 2 ? tile[ct - 2 * n] : Z;
-      const Sp<R> yp1 = y + 1 < n ? tile[ct + n] : Z;
-      const Sp<R> yp2 = y + 2 < n ? tile[ct + 2 * n] : Z;
+      // y neighbours need no test: rows outside the plane are the zero
+      // halo slots (hq = -1) or centre slots of threads without a point
+      const Sp<R> ym1 = tile[ct - n];
+      const Sp<R> ym2 = tile[ct - 2 * n];
+      const Sp<R> yp1 = tile[ct + n];
+      const Sp<R> yp2 = tile[ct + 2 * n];
       const Sp<R> a0 = z00;
       Sp<R> d[3];
 #define NHF_D(dd, P1, M1, P2, M2)                                  \
@@ -265,8 +307,6 @@ __global__ __launch_bounds__(BS) void k_hpsi25(const Sp<R>* __restrict__ in, Sp<
       lap.di = (R(16) * (xp1.di + xm1.di + yp1.di + ym1.di + zp1.di + zm1.di) -
                 (xp2.di + xm2.di + yp2.di + ym2.di + zp2.di + zm2.di) - R(90) * a0.di) * c2;
       const size_t p = (size_t)z * nn + q;
-      R F[NFLD];
-      load_fields(fld, p, F);
       const R f = F[0], gx = F[1], gy = F[2], gz = F[3], U = F[4] + shift, Wx = F[5], Wy = F[6], Wz = F[7];
       const R t0r = -Wy * d[0].ur + Wz * d[0].di + Wx * d[1].ur - Wz * d[1].dr + Wy * d[2].dr - Wx * d[2].di;
       const R t0i = -Wy * d[0].ui - Wz * d[0].dr + Wx * d[1].ui - Wz * d[1].di + Wy * d[2].di + Wx * d[2].dr;
@@ -288,6 +328,11 @@ __global__ __launch_bounds__(BS) void k_hpsi25(const Sp<R>* __restrict__ in, Sp<
     zm1 = z00;
     z00 = zp1;
     zp1 = zp2;
+    zp2 = zp3;
+    hcur = hnext;
+    if (own && z + 1 < z1)
+#pragma unroll
+      for (int k = 0; k < NFLD; ++k) F[k] = Fn[k];
   }
   (void)N3;
   if (DOT) block_atomic_add<2>(acc, dot + (size_t)dot_stride * c);
@@ -301,21 +346,22 @@ void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int nco
   R c1 = (R)(1.0 / (12.0 * h)), c2 = (R)(1.0 / (12.0 * h * h));
   hipEvent_t e0 = ev ? ev->start : nullptr, e1 = ev ? ev->stop : nullptr;
   if (4 * n <= BS && hpsi_variant() == 1) {
-    // z chunks: enough workgroups to fill the device (>= ~2048), but no
-    // chunk shorter than 4 planes (4 warm-up loads per chunk)
+    // z chunks: enough workgroups to fill the device (about hpsi_wg_target()),
+    // but no chunk shorter than 4 planes (4 warm-up loads per chunk)
     const int ntile = (n * n + BS - 1) / BS;
-    int nzc = (2048 + ntile * ncols - 1) / (ntile * ncols);
+    int nzc = (hpsi_wg_target() + ntile * ncols - 1) / (ntile * ncols);
     nzc = std::max(1, std::min(nzc, n / 4));
     const int zc = (n + nzc - 1) / nzc;
     nzc = (n + zc - 1) / zc;
-    dim3 g3((unsigned)ntile, (unsigned)nzc, (unsigned)ncols);
+    const int total = ntile * nzc * ncols, xcd = hpsi_xcd();
+    dim3 g1((unsigned)(xcd ? (total + 7) / 8 * 8 : total));
     size_t sh = sizeof(Sp<R>) * (BS + 4 * n);
     if (dot)
-      LAUNCH_EV(e0, e1, (k_hpsi25<R, true>), g3, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
-                dot_stride, done, act, clist);
+      LAUNCH_EV(e0, e1, (k_hpsi25<R, true>), g1, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
+                dot_stride, done, act, clist, ntile, nzc, ncols, xcd);
     else
-      LAUNCH_EV(e0, e1, (k_hpsi25<R, false>), g3, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
-                dot_stride, done, act, clist);
+      LAUNCH_EV(e0, e1, (k_hpsi25<R, false>), g1, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
+                dot_stride, done, act, clist, ntile, nzc, ncols, xcd);
     return;
   }
   dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
