@@ -13,11 +13,14 @@
  *
  * Ownership: input pointers are caller-owned and read only during the call;
  * outputs are caller-allocated.  The context owns every device buffer.
- * Threading: one host thread per context; all work is enqueued on the
- * context's own HIP stream.  Errors are negative integers (NHF_E*).
+ * Threading: one host thread per context; each problem enqueues its work on
+ * its own HIP streams (nhf_batch_run drives several problems from threads
+ * of its own).  Errors are negative integers (NHF_E*).
  */
 #ifndef NHF_H
 #define NHF_H
+
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -83,7 +86,7 @@ typedef struct nhf_gcg_params {
 #define NHF_CON_XY2 4
 #define NHF_CON_OCT 5
 #define NHF_CON_Q20 6
-#define NHF_MAX_CON 8
+#define NHF_MAX_CON 12
 
 typedef struct nhf_constraint {
   int kind;        /* NHF_CON_* */
@@ -239,6 +242,38 @@ int nhf_gcg_step(void *ctx, int pid, int q, const double *conjdir, int p_cols,
 int nhf_scf_run(void *ctx, int pid, double *psi_n, double *psi_p, double *eps_n,
                 double *eps_p, nhf_result *res, double *trace, int trace_cap);
 
+/* Deformation curve (main.cpp:122-191 and the HF loop to :366): npts
+ * self-consistent runs one after the other, each started from the previous
+ * point's states, with the target of the problem's LAST constraint (which
+ * must be NHF_CON_Q20) set to mu20[i] (main.cpp:186-189).  As in the
+ * reference, the density-mixing history, the pairing state and the
+ * constraint multipliers carry over from point to point; the conjugate
+ * directions and the energy history restart.  The reference builds the
+ * list as COM x/y/z, x^2-y^2, 2xy, octupole (target 0) then Q20
+ * (main.cpp:175-182) with mu20 = beta 3 A R^2 / 4 pi, R = 1.2 A^(1/3)
+ * (utilities.cpp:71-73).  psi/eps as nhf_scf_run (the last point's states
+ * out); res: npts results. */
+int nhf_curve_run(void *ctx, int pid, const double *mu20, int npts, double *psi_n, double *psi_p,
+                  double *eps_n, double *eps_p, nhf_result *res);
+
+/* One item of nhf_batch_run: the arguments of nhf_scf_run for problem pid;
+ * rc receives that run's return code. */
+typedef struct nhf_batch_item {
+  int pid;
+  double *psi_n, *psi_p, *eps_n, *eps_p;
+  nhf_result *res;
+  double *trace;
+  int trace_cap;
+  int rc;
+} nhf_batch_item;
+
+/* Independent problems (deformation points, nuclei, seeds) run concurrently
+ * on the one device, each as nhf_scf_run with its own streams.  No
+ * counterpart in the reference, which runs its problems one after the other
+ * (main.cpp:46-53, 151-366).  Each pid at most once per call.  Returns the
+ * first non-zero item rc, else NHF_OK. */
+int nhf_batch_run(void *ctx, nhf_batch_item *items, int count);
+
 /* Measurement: average duration (ms, HIP events on the context stream) of
  * `reps` back-to-back H.psi stencil launches over m device-resident states
  * of species q in precision prec (must equal the problem precision).
@@ -277,6 +312,42 @@ int nhf_block_gram(void *ctx, size_t npts, int ka, const double *A, int kb, cons
  * (fm), Euclidean-normalised columns (guess.cpp:10-248 with
  * useSpinOrbit = true).  Host-side; out: 2n^3 x nev complex. */
 int nhf_ho_guess(int n, double box_size, int nev, double b, double *out);
+
+/* Woods-Saxon start-up parameters (reference parameters/woods_saxon.json as
+ * read at input_parser.cpp:54-71): central depth V0 (MeV), r0 (fm),
+ * diffusivity (fm), isospin factor kappa, spin-orbit depth V0_so (MeV).  The
+ * spin-orbit term uses the central radius and diffusivity (main.cpp:78-84).
+ * NULL where a pointer is accepted = those file values:
+ * {55.5, 1.20, 0.67, 0.86, 55.5}. */
+typedef struct nhf_ws_params {
+  double V0, r0, diff, kappa, V0_so;
+} nhf_ws_params;
+
+/* Fields of the deformed Woods-Saxon start-up Hamiltonian in the layout of
+ * nhf_set_fields: f = hbar^2/2m (the kinetic term of build_matrix5p,
+ * hamiltonian.cpp:228-230), grad f = 0, U = DeformedWoodsSaxonPotential
+ * (deformed_woods_saxon.cpp:15-20) with the radius R0 (1 + beta2 Y20) of
+ * radius.cpp:8-12 and the beta3 R0 Y30 shift, W = ls grad V of
+ * DeformedSpinOrbitPotential (deformed_spin_orbit.cpp:12-35,68-114;
+ * ls = -35 (hbar/2m)^2), zero when spin_orbit is 0.  The radius gradient
+ * keeps the reference's Y20Grad (spherical_harmonics.cpp:174-188), whose
+ * vector starts as setIdentity() = (1,0,0): only its x component,
+ * 6 sqrt(5/16pi) x z^2 / r^3, is non-zero.  Host-side; f, U: n^3,
+ * gradf, W: 3 n^3. */
+int nhf_ws_fields(int n, double box_size, int A, int Z, double beta2, double beta3, int spin_orbit,
+                  const nhf_ws_params *p, double *f, double *gradf, double *U, double *W);
+
+/* Woods-Saxon start-up states (main.cpp:72-116): the oscillator guess of
+ * max(M_n, M_p) columns with length 1.2 A^0.3333333 fm, nhf_ws_fields with
+ * the problem's spin_orbit flag, one block-GCG call with the start-up
+ * parameters (NULL gcg = parameters/woods_saxon.json: max_iter 10, steps 40,
+ * tol 1e-2, cg_tol 1e-4), then the first M_n / M_p eigenvectors for each
+ * species, trapezoid-normalised (wavefunction.cpp:362-370), and their
+ * eigenvalues.  Runs on the device in the problem's precision and leaves
+ * the Woods-Saxon fields in the species slot it used (nhf_scf_run computes
+ * its own).  psi_n / psi_p: 2n^3 x M_n / M_p out; eps_n / eps_p: out. */
+int nhf_ws_guess(void *ctx, int pid, double beta2, double beta3, const nhf_ws_params *p,
+                 const nhf_gcg_params *gcg, double *psi_n, double *psi_p, double *eps_n, double *eps_p);
 
 /* Host FP64 eigensolver of the Rayleigh-Ritz step: eigenpairs of the
  * Hermitian matrix defined by the LOWER triangle of A (k x k complex,

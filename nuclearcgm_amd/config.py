@@ -27,7 +27,7 @@ MIX_LINEAR, MIX_BROYDEN = 0, 1
 W_JACOBI_CG, W_LEARNED = 0, 1
 PAIR_NONE, PAIR_BCS, PAIR_HFB = 0, 1, 2
 CON_XCM, CON_YCM, CON_ZCM, CON_X2MY2, CON_XY2, CON_OCT, CON_Q20 = range(7)
-MAX_CON = 8
+MAX_CON = 12
 NTERMS = 19
 TERM_NAMES = ["C0rho", "C1rho", "C0lap", "C1lap", "C0tau", "C1tau", "C0J2", "C1J2", "C0divJ", "C1divJ",
               "coul_dir", "coul_ex", "kin", "pair_n", "pair_p", "rear", "constr", "E_int", "E_HF"]
@@ -123,6 +123,83 @@ def load_functional(name):
     keys = ["C0rr", "C1rr", "C0Drr", "C1Drr", "C0rt", "C1rt", "C0J2", "C1J2", "C0rdr", "C1rdr", "C0nJ", "C1nJ",
             "sigma"]
     return [par[k] for k in keys], par
+
+
+class NhfBatchItem(ctypes.Structure):
+    """One problem of nhf_batch_run."""
+    _fields_ = [("pid", ctypes.c_int), ("psi_n", ctypes.POINTER(ctypes.c_double)),
+                ("psi_p", ctypes.POINTER(ctypes.c_double)), ("eps_n", ctypes.POINTER(ctypes.c_double)),
+                ("eps_p", ctypes.POINTER(ctypes.c_double)), ("res", ctypes.POINTER(NhfResult)),
+                ("trace", ctypes.POINTER(ctypes.c_double)), ("trace_cap", ctypes.c_int), ("rc", ctypes.c_int)]
+
+
+class NhfWsParams(ctypes.Structure):
+    """Woods-Saxon start-up parameters (nhf_ws_params)."""
+    _fields_ = [("V0", ctypes.c_double), ("r0", ctypes.c_double), ("diff", ctypes.c_double),
+                ("kappa", ctypes.c_double), ("V0_so", ctypes.c_double)]
+
+
+def ws_params():
+    """parameters/woods_saxon.json of the reference (input_parser.cpp:54-71)."""
+    return NhfWsParams(V0=55.5, r0=1.20, diff=0.67, kappa=0.86, V0_so=55.5)
+
+
+def ws_gcg_params():
+    """The start-up block-GCG parameters of parameters/woods_saxon.json."""
+    return NhfGcgParams(max_iter=10, steps=40, tol=1e-2, cg_tol=1e-4)
+
+
+def initial_deformation(data):
+    """(beta2, beta3) of the start-up potential (input_parser.cpp:202-218):
+    beta2 = deformation.guess (default deformation.start) for a deformation
+    curve, else initialBeta2 (default 0); beta3 default 0."""
+    if "deformation" in data:
+        d = data["deformation"]
+        beta2 = float(d.get("guess", d["start"]))
+    else:
+        beta2 = float(data.get("initialBeta2", 0.0))
+    return beta2, float(data.get("beta3", 0.0))
+
+
+def mu20_from_beta(beta, A):
+    """Utilities::mu20FromBeta (utilities.cpp:71-73) with R = 1.2 A^(1/3)
+    (main.cpp:127): mu20 = beta 3 A R^2 / (4 pi)."""
+    R = 1.2 * A ** (1.0 / 3.0)
+    return beta * 3 * A * R * R / 4 / math.pi
+
+
+def curve_targets(start, end, step, A):
+    """The mu20 targets of a deformation curve (main.cpp:127-144): start to
+    end in steps of mu20(step), accumulated in floating point as the
+    reference's loop does."""
+    s, e, d = (mu20_from_beta(b, A) for b in (start, end, step))
+    out = []
+    mu = s
+    if s < e:
+        while mu <= e:
+            out.append(mu)
+            mu += d
+    else:
+        while mu >= e:
+            out.append(mu)
+            mu -= d
+    return out
+
+
+def set_curve_constraints(cfg, mu20, c_q20=0.005):
+    """The constraint list of a deformation curve (main.cpp:175-182): COM
+    x, y, z (stiffness 0.05), x^2-y^2 and 2xy (0.005), octupole (0.001), all
+    with target 0, then Q20 (0.005 in the reference; c_q20 here) with
+    target mu20.  Keeps constrainCOM's entries (main.cpp:168-173) in front."""
+    cons = [(cfg.con[i].kind, cfg.con[i].target, cfg.con[i].C) for i in range(cfg.n_con)]
+    cons += [(CON_XCM, 0.0, 0.05), (CON_YCM, 0.0, 0.05), (CON_ZCM, 0.0, 0.05), (CON_X2MY2, 0.0, 0.005),
+             (CON_XY2, 0.0, 0.005), (CON_OCT, 0.0, 0.001), (CON_Q20, mu20, c_q20)]
+    if len(cons) > MAX_CON:
+        raise ValueError("too many constraints")
+    cfg.n_con = len(cons)
+    for i, (k, t, c) in enumerate(cons):
+        cfg.con[i] = NhfConstraint(k, t, c)
+    return cfg
 
 
 def hf_gcg_params():

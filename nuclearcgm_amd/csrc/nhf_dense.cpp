@@ -327,6 +327,16 @@ int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, s
   return ke;
 }
 
+// A real symmetric matrix stays real through the Householder reduction and
+// the inverse iteration of eigh_lower: its eigenvectors come out real.
+bool eigh_real_sym(int k, const double* A, double* w, double* V) {
+  std::vector<cd> Ac((size_t)k * k), Vc((size_t)k * k);
+  for (size_t i = 0; i < (size_t)k * k; ++i) Ac[i] = A[i];
+  if (!eigh_lower(k, Ac.data(), w, Vc.data(), k)) return false;
+  for (size_t i = 0; i < (size_t)k * k; ++i) V[i] = Vc[i].real();
+  return true;
+}
+
 }  // namespace nhf
 
 // ---------------------------------------------------------------- C ABI

@@ -1,4 +1,5 @@
-// nhf_guess.cpp -- harmonic-oscillator spinor start-up states.
+// nhf_guess.cpp -- start-up states and fields on the host: the
+// harmonic-oscillator spinor guess and the deformed Woods-Saxon fields.
 //
 // Spin-orbit-coupled |n_r l j m_j> states ordered by major shell N, then
 // n_r, then j = l + 1/2 before j = l - 1/2, |m_j| ascending with +m_j before
@@ -110,5 +111,55 @@ extern "C" int nhf_ho_guess(int n, double box_size, int nev, double b, double* o
     const double inv = 1.0 / std::sqrt(nsq);
     for (size_t r = 0; r < rows; ++r) col[r] *= inv;
   }
+  return NHF_OK;
+}
+
+// Deformed Woods-Saxon start-up fields.  Central: -V0 (1 + kappa (A-2Z)/A)
+// / (1 + exp((r - R(theta) - beta3 R0 Y30(theta)) / a)) with
+// R(theta) = R0 (1 + beta2 Y20(theta)), R0 = r0 A^(1/3).  Spin-orbit:
+// W = ls g, g = V0_so / a * e^t / (1 + e^t)^2 * (r_hat - grad R) with
+// t = (r - R(theta)) / a (no beta3 shift), ls = -35 (hbar / 2m)^2, and the
+// reference's grad R = R0 beta2 (6 sqrt(5/16pi) x z^2 / r^3, 0, 0).
+// e^t / (1 + e^t)^2 is evaluated as e^-|t| / (1 + e^-|t|)^2 (the function
+// is even in t), which cannot overflow.
+extern "C" int nhf_ws_fields(int n, double box_size, int A, int Z, double beta2, double beta3, int spin_orbit,
+                             const nhf_ws_params* p, double* f, double* gradf, double* U, double* W) {
+  static const nhf_ws_params file_values = {55.5, 1.20, 0.67, 0.86, 55.5};
+  if (n < 2 || !(box_size > 0) || A < 1 || Z < 0 || Z > A || !f || !gradf || !U || !W) return NHF_EINVAL;
+  const nhf_ws_params& P = p ? *p : file_values;
+  if (!(P.diff > 0) || !(P.r0 > 0)) return NHF_EINVAL;
+  const double hbar = 197.327053, mnuc = 938.91869;
+  const double a = 0.5 * box_size, h = 2.0 * a / (n - 1);
+  const size_t N3 = (size_t)n * n * n;
+  const double R0 = P.r0 * std::pow((double)A, 1.0 / 3.0);
+  const double depth = P.V0 * (1.0 + P.kappa * (A - 2 * Z) / (double)A);
+  const double c20 = std::sqrt(5.0 / (16.0 * PI)), c30 = std::sqrt(7.0 / (4.0 * PI));
+  const double hb2m = hbar * hbar / (2.0 * mnuc);
+  const double ls = -35.0 * (0.5 * hbar / mnuc) * (0.5 * hbar / mnuc);
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const size_t q = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k);
+        const double x = -a + i * h, y = -a + j * h, z = -a + k * h;
+        const double r = std::sqrt(x * x + y * y + z * z);
+        f[q] = hb2m;
+        gradf[q] = gradf[N3 + q] = gradf[2 * N3 + q] = 0.0;
+        W[q] = W[N3 + q] = W[2 * N3 + q] = 0.0;
+        if (r == 0.0) {  // the centre (odd meshes only): the spherical limit
+          U[q] = -depth / (1.0 + std::exp(-R0 / P.diff));
+          continue;
+        }
+        const double ct = z / r;
+        const double Rt = R0 * (1.0 + beta2 * c20 * (3.0 * ct * ct - 1.0));
+        const double y30 = c30 * 0.5 * ct * (5.0 * ct * ct - 3.0);
+        U[q] = -depth / (1.0 + std::exp((r - Rt - beta3 * R0 * y30) / P.diff));
+        if (!spin_orbit) continue;
+        const double e = std::exp(-std::fabs((r - Rt) / P.diff));
+        const double g = ls * P.V0_so / P.diff * e / ((1.0 + e) * (1.0 + e));
+        const double dRx = R0 * beta2 * 6.0 * c20 * x * z * z / (r * r * r);
+        W[q] = g * (x / r - dRx);
+        W[N3 + q] = g * (y / r);
+        W[2 * N3 + q] = g * (z / r);
+      }
   return NHF_OK;
 }

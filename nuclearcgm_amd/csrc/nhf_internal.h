@@ -188,4 +188,16 @@ void launch_broyden_update(hipStream_t st, size_t D, double* x, const double* F,
                            const double* hx, int L, unsigned valid, const BroydenCoef& g, double alpha,
                            size_t nclamp);
 
+// two-basis HFB: time-reversal completion of a basis, kappa(r) from
+// chi = ph K^T, and the pair-field columns B whose Gram with ph gives Delta
+template <typename R>
+void launch_numerical_tr(hipStream_t st, const Sp<R>* X, Sp<R>* Y, size_t N3, size_t ld, int ncols);
+template <typename R>
+void launch_hfb_kappa(hipStream_t st, const Sp<R>* ph, const Sp<R>* chi, int ns, size_t N3, size_t ld, double* kr);
+template <typename R>
+void launch_hfb_pairfield(hipStream_t st, const Sp<R>* ph, const double* kr, int n, double h, size_t ld, int ncols,
+                          Sp<R>* B);
+// real symmetric eigenproblem (lower triangle), ascending; V real k x k
+bool eigh_real_sym(int k, const double* A, double* w, double* V);
+
 }  // namespace nhf

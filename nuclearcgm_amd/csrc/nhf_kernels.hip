@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 #include <type_traits>
 
@@ -20,6 +21,25 @@ namespace nhf {
 
 constexpr int BS = 256;
 
+// NHF_GRAM=v1: the previous split-BF16 Gram kernel (one 32 x 32 block per
+// workgroup, 16x16x16 MFMA), kept for comparison
+static bool gram_v1() {
+  static bool v = [] {
+    const char* e = std::getenv("NHF_GRAM");
+    return e && std::strcmp(e, "v1") == 0;
+  }();
+  return v;
+}
+// workgroups the LDS-staged Gram aims for (NHF_GRAM_WG, default 256: one
+// 12-wavefront workgroup per CU)
+static int gram_wg_target() {
+  static int v = [] {
+    const char* e = std::getenv("NHF_GRAM_WG");
+    const int x = e ? std::atoi(e) : 0;
+    return x > 0 ? x : 256;
+  }();
+  return v;
+}
 // FP32 block products: split-BF16 MFMA Gram and FP32-accumulating rotation
 // (default) or the FP64-accumulating kernels (environment NHF_BLOCK=fp64)
 static bool block_fp64() {
@@ -782,6 +802,195 @@ __global__ __launch_bounds__(BS) void k_gram_mfma(const Sp<float>* __restrict__ 
       }
 }
 
+
+// ---- LDS-staged split-BF16 block product of FP32 states
+// (v_mfma_f32_16x16x32_bf16).  A workgroup owns a GS x GS output super-tile
+// (rows I0.., columns J0..) over one chunk of points, so each column of A and
+// B is read from HBM once per chunk.  Per slab of 8 points (K = 32 real
+// components) the workgroup splits the super-tile's columns exactly into
+// three BF16 parts, x = h + m + l (split2), and stages them in LDS; each
+// wavefront then forms the six products
+// hh' + hm' + mh' + hl' + lh' + mm' of its 16 x 16 tiles (the dropped terms
+// are below 2^-24 relative).  Layout of an LDS unit (16 B = 8 BF16): lane
+// group g = lane / 16 holds points 2g, 2g + 1 of the slab, components
+// (ur, ui, dr, di) -- the MFMA operand order A[l % 16][8 (l / 16) + r],
+// B[8 (l / 16) + r][l % 16]; D[4 (l / 16) + r][l % 16].
+// Work per wavefront (12 per workgroup): one tile row and three tile
+// columns of the 6 x 6 tiles of a super-tile (rows w / 2, columns
+// 3 (w % 2)..); in the diagonal super-tile of a Hermitian product only the
+// lower tile triangle, as nine row triples.  FP32 accumulators are flushed
+// into FP64 sums every two slabs; the FP64 chunk partials are summed by
+// k_reduce_parts (FP32 partials would add a rounding of each chunk sum,
+// 6e-8 of it, which the sum over few chunks does not average out).
+typedef __bf16 gbf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int gu32x4 __attribute__((ext_vector_type(4)));
+constexpr int GS = 96;   // super-tile edge
+constexpr int GSU = 5;   // LDS units per column and split level (4 + 1 pad)
+constexpr int GUNITS = 2 * 3 * GS * GSU;
+constexpr int GWAVES = 12;
+constexpr int GBS = 64 * GWAVES;
+typedef __bf16 gbf16x2 __attribute__((ext_vector_type(2)));
+typedef float gf2v __attribute__((ext_vector_type(2)));
+// two FP32 values rounded to nearest-even BF16, packed (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned cvt_pk_bf16(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(gf2v{a, b}, gbf16x2));
+}
+// exact three-way split of two FP32 values into packed BF16 pairs: each
+// remainder is exact in FP32 and the last one has at most 8 significant
+// bits, so x = h + m + l exactly; rounding to nearest keeps |m| <= 2^-8 |x|
+// and |l| <= 2^-16 |x| with either sign, so the dropped products ml, lm, ll
+// stay below 2^-24 relative and do not accumulate a bias
+__device__ __forceinline__ void split2(float x0, float x1, unsigned& h, unsigned& m, unsigned& l) {
+  h = cvt_pk_bf16(x0, x1);
+  const float r0 = x0 - __uint_as_float(h << 16), r1 = x1 - __uint_as_float(h & 0xffff0000u);
+  m = cvt_pk_bf16(r0, r1);
+  const float s0 = r0 - __uint_as_float(m << 16), s1 = r1 - __uint_as_float(m & 0xffff0000u);
+  l = cvt_pk_bf16(s0, s1);
+}
+// (re, im) pairs -> (im, -re): multiplication by -i of packed BF16 complexes
+__device__ __forceinline__ gu32x4 mul_mi8(const gu32x4& v) {
+  gu32x4 o;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) o[r] = __builtin_amdgcn_alignbit(v[r], v[r], 16) ^ 0x80000000u;
+  return o;
+}
+__device__ __forceinline__ void mfma32(gf32x4& acc, const gu32x4& a, const gu32x4& b) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(gbf16x8, a), __builtin_bit_cast(gbf16x8, b),
+                                                acc, 0, 0, 0);
+}
+// the twelve products of one tile and one slab (smallest terms first)
+__device__ __forceinline__ void tile12(gf32x4& re, gf32x4& im, const gu32x4 (&a)[3], const gu32x4 (&b)[3],
+                                       const gu32x4 (&bx)[3]) {
+  mfma32(re, a[1], b[1]);
+  mfma32(re, a[2], b[0]);
+  mfma32(re, a[0], b[2]);
+  mfma32(re, a[1], b[0]);
+  mfma32(re, a[0], b[1]);
+  mfma32(re, a[0], b[0]);
+  mfma32(im, a[1], bx[1]);
+  mfma32(im, a[2], bx[0]);
+  mfma32(im, a[0], bx[2]);
+  mfma32(im, a[1], bx[0]);
+  mfma32(im, a[0], bx[1]);
+  mfma32(im, a[0], bx[0]);
+}
+__global__ __launch_bounds__(GBS) void k_gram_lds(const Sp<float>* __restrict__ A, size_t lda, int ka,
+                                                  const Sp<float>* __restrict__ B, size_t ldb, int kb, size_t N3,
+                                                  int chunk_pts, double2* __restrict__ part, int lower) {
+  __shared__ gu32x4 lds[GUNITS];
+  const int nsj = (kb + GS - 1) / GS;
+  const int I0 = (blockIdx.x / nsj) * GS, J0 = (blockIdx.x % nsj) * GS;
+  if (lower && I0 < J0) return;  // super-tile strictly above the diagonal
+  const bool diag = lower && I0 == J0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, c = lane & 15;
+  const size_t p0 = (size_t)blockIdx.y * chunk_pts;
+  const size_t p1 = min(N3, p0 + (size_t)chunk_pts);
+  // staging units: (matrix, column, group); the diagonal super-tile needs A only
+  const int nunits = (diag ? 1 : 2) * GS * 4;
+  // this wavefront's tile row and first tile column
+  int trow, tc0;
+  bool act;
+  if (!diag) {
+    trow = wave >> 1;
+    tc0 = 3 * (wave & 1);
+    act = true;
+  } else {
+    // lower tile triangle as row triples: (0,0) (1,0) (2,0) (3,0) (3,3) (4,0) (4,3) (5,0) (5,3)
+    const int rows[9] = {0, 1, 2, 3, 3, 4, 4, 5, 5}, cols[9] = {0, 0, 0, 0, 3, 0, 3, 0, 3};
+    act = wave < 9;
+    trow = act ? rows[wave] : 0;
+    tc0 = act ? cols[wave] : 0;
+  }
+  act = act && I0 + 16 * trow < ka;
+  bool on[3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) on[s] = act && J0 + 16 * (tc0 + s) < kb && (!diag || tc0 + s <= trow);
+  gf32x4 re[3], im[3];
+  double dre[3][4], dim[3][4];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    re[s] = gf32x4{0.f, 0.f, 0.f, 0.f};
+    im[s] = gf32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dre[s][r] = dim[s][r] = 0.0;
+  }
+  // register prefetch of one slab: one unit per thread, two points
+  float4 pf[2];
+  const bool has_unit = tid < nunits;
+  const int umat = tid / (GS * 4), ucol = (tid >> 2) % GS, ugrp = tid & 3;
+  const int ugc = (umat ? J0 : I0) + ucol;
+  const bool ucol_ok = has_unit && ugc < (umat ? kb : ka);
+  const float4* usrc = reinterpret_cast<const float4*>((umat ? B : A) + (size_t)(ucol_ok ? ugc : 0) * (umat ? ldb : lda));
+  auto fetch = [&](size_t base) {
+    pf[0] = pf[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const size_t p = base + 2 * ugrp;
+    if (ucol_ok) {
+      if (p < p1) pf[0] = usrc[p];
+      if (p + 1 < p1) pf[1] = usrc[p + 1];
+    }
+  };
+  const int bmat = diag ? 0 : 1;
+  const int ua = (16 * trow + c) * GSU + g;
+  const int ub0 = ((bmat * 3) * GS + 16 * tc0 + c) * GSU + g;
+  fetch(p0);
+  int ns = 0;
+  for (size_t base = p0; base < p1; base += 8) {
+    __syncthreads();  // the previous slab's operands are consumed
+    if (has_unit) {
+      unsigned hh[4], mm[4], ll[4];
+      split2(pf[0].x, pf[0].y, hh[0], mm[0], ll[0]);
+      split2(pf[0].z, pf[0].w, hh[1], mm[1], ll[1]);
+      split2(pf[1].x, pf[1].y, hh[2], mm[2], ll[2]);
+      split2(pf[1].z, pf[1].w, hh[3], mm[3], ll[3]);
+      const int ud = ((umat * 3) * GS + ucol) * GSU + ugrp;
+      lds[ud] = gu32x4{hh[0], hh[1], hh[2], hh[3]};
+      lds[ud + GS * GSU] = gu32x4{mm[0], mm[1], mm[2], mm[3]};
+      lds[ud + 2 * GS * GSU] = gu32x4{ll[0], ll[1], ll[2], ll[3]};
+    }
+    if (base + 8 < p1) fetch(base + 8);
+    __syncthreads();
+    if (act) {
+      gu32x4 a[3];
+#pragma unroll
+      for (int v = 0; v < 3; ++v) a[v] = lds[ua + v * GS * GSU];
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        if (!on[s]) continue;
+        gu32x4 b[3], bx[3];
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+          b[v] = lds[ub0 + 16 * s * GSU + v * GS * GSU];
+          bx[v] = mul_mi8(b[v]);
+        }
+        tile12(re[s], im[s], a, b, bx);
+      }
+    }
+    if (++ns == 2) {
+      ns = 0;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          dre[s][r] += re[s][r];
+          dim[s][r] += im[s][r];
+        }
+        re[s] = gf32x4{0.f, 0.f, 0.f, 0.f};
+        im[s] = gf32x4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+  }
+  double2* out = part + (size_t)blockIdx.y * ka * kb;
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    if (!on[s]) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = I0 + 16 * trow + 4 * g + r, j = J0 + 16 * (tc0 + s) + c;
+      if (i < ka && j < kb)
+        out[(size_t)j * ka + i] = make_double2(dre[s][r] + re[s][r], dim[s][r] + im[s][r]);
+    }
+  }
+}
 // chunk length (points) for the split Gram: enough workgroups to fill the
 // device (about 1024), a multiple of 16, at least 256 points
 static int gram_mfma_chunk(size_t N3, int blocks) {
@@ -796,6 +1005,24 @@ void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>
   lower = (lower && A == B && ka == kb) ? 1 : 0;
   size_t cnt = (size_t)ka * kb;
   hipEvent_t e0 = ev ? ev->start : nullptr, e1 = ev ? ev->stop : nullptr, none = nullptr;
+  if (std::is_same<R, float>::value && !block_fp64() && !gram_v1()) {
+    const int nsi = (ka + GS - 1) / GS, nsj = (kb + GS - 1) / GS;
+    const int active = lower ? nsi * (nsi + 1) / 2 : nsi * nsj;
+    const size_t want = (size_t)std::max(1, gram_wg_target() / active);
+    size_t cp = (N3 + want - 1) / want;
+    cp = (cp + 7) / 8 * 8;
+    size_t nc = (N3 + cp - 1) / cp;
+    if (nc > (size_t)nchunk) {  // partial-sum slots of the caller's buffer
+      nc = (size_t)nchunk;
+      cp = ((N3 + nc - 1) / nc + 7) / 8 * 8;
+      nc = (N3 + cp - 1) / cp;
+    }
+    LAUNCH_EV(e0, none, k_gram_lds, dim3(nsi * nsj, (unsigned)nc), dim3(GBS), 0, st, (const Sp<float>*)A, lda, ka,
+              (const Sp<float>*)B, ldb, kb, N3, (int)cp, part, lower);
+    LAUNCH_EV(none, e1, k_reduce_parts, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, (int)nc, cnt, ka,
+              lower ? 16 : 0, C);
+    return;
+  }
   if (std::is_same<R, float>::value && !block_fp64()) {
     const int nbi = (ka + GB - 1) / GB, nbj = (kb + GB - 1) / GB;
     const int cp = gram_mfma_chunk(N3, nbi * nbj);
@@ -1073,6 +1300,79 @@ void launch_time_reverse(hipStream_t st, const Sp<R>* X, Sp<R>* Y, size_t count)
 }
 template void launch_time_reverse<double>(hipStream_t, const Sp<double>*, Sp<double>*, size_t);
 template void launch_time_reverse<float>(hipStream_t, const Sp<float>*, Sp<float>*, size_t);
+
+// ======================================================== two-basis HFB
+// numericalTR (iteration_data.cpp:580-589): odd columns replaced by the time
+// reverse of the preceding even column
+template <typename R>
+__global__ void k_numerical_tr(const Sp<R>* __restrict__ X, Sp<R>* __restrict__ Y, size_t N3, size_t ld) {
+  const int c = blockIdx.y;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  if (c & 1) {
+    const Sp<R> v = X[(size_t)(c - 1) * ld + p];
+    Y[(size_t)c * ld + p] = Sp<R>{v.dr, -v.di, -v.ur, v.ui};
+  } else {
+    Y[(size_t)c * ld + p] = X[(size_t)c * ld + p];
+  }
+}
+template <typename R>
+void launch_numerical_tr(hipStream_t st, const Sp<R>* X, Sp<R>* Y, size_t N3, size_t ld, int ncols) {
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  hipLaunchKernelGGL((k_numerical_tr<R>), grid, dim3(BS), 0, st, X, Y, N3, ld);
+}
+template void launch_numerical_tr<double>(hipStream_t, const Sp<double>*, Sp<double>*, size_t, size_t, int);
+template void launch_numerical_tr<float>(hipStream_t, const Sp<float>*, Sp<float>*, size_t, size_t, int);
+
+// kappa(r) = -Re sum_a [ph_a(up) chi_a(down) - ph_a(down) chi_a(up)] with
+// chi_a = sum_b f_a f_b kappa_ab ph_b (kappa_field, iteration_data.cpp:467-501)
+template <typename R>
+__global__ __launch_bounds__(BS) void k_hfb_kappa(const Sp<R>* __restrict__ ph, const Sp<R>* __restrict__ chi, int ns,
+                                                  size_t N3, size_t ld, double* __restrict__ kr) {
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  double s = 0.0;
+  for (int a = 0; a < ns; ++a) {
+    const Sp<R> u = ph[(size_t)a * ld + p], x = chi[(size_t)a * ld + p];
+    // Re[(u.u)(x.d) - (u.d)(x.u)]
+    s += ((double)u.ur * x.dr - (double)u.ui * x.di) - ((double)u.dr * x.ur - (double)u.di * x.ui);
+  }
+  kr[p] = -s;
+}
+template <typename R>
+void launch_hfb_kappa(hipStream_t st, const Sp<R>* ph, const Sp<R>* chi, int ns, size_t N3, size_t ld, double* kr) {
+  hipLaunchKernelGGL((k_hfb_kappa<R>), dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, ph, chi, ns, N3, ld, kr);
+}
+template void launch_hfb_kappa<double>(hipStream_t, const Sp<double>*, const Sp<double>*, int, size_t, size_t, double*);
+template void launch_hfb_kappa<float>(hipStream_t, const Sp<float>*, const Sp<float>*, int, size_t, size_t, double*);
+
+// B_b(r) = w(r) h^3 kappa(r) (conj(ph_b(down)), -conj(ph_b(up))): then
+// (ph^H B)_ab = int conj(ph_a(up) ph_b(down) - ph_a(down) ph_b(up)) kappa
+// (delta_matrix, iteration_data.cpp:503-538, trapezoid weights w)
+template <typename R>
+__global__ void k_hfb_pairfield(const Sp<R>* __restrict__ ph, const double* __restrict__ kr, int n, double h3,
+                                size_t ld, Sp<R>* __restrict__ B) {
+  const int c = blockIdx.y;
+  const size_t N3 = (size_t)n * n * n;
+  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= N3) return;
+  int i, j, k;
+  decompose(p, n, i, j, k);
+  const double g = trap_w(i, j, k, n) * h3 * kr[p];
+  const Sp<R> v = ph[(size_t)c * ld + p];
+  B[(size_t)c * ld + p] = Sp<R>{(R)(g * v.dr), (R)(-g * v.di), (R)(-g * v.ur), (R)(g * v.ui)};
+}
+template <typename R>
+void launch_hfb_pairfield(hipStream_t st, const Sp<R>* ph, const double* kr, int n, double h, size_t ld, int ncols,
+                          Sp<R>* B) {
+  const size_t N3 = (size_t)n * n * n;
+  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  hipLaunchKernelGGL((k_hfb_pairfield<R>), grid, dim3(BS), 0, st, ph, kr, n, h * h * h, ld, B);
+}
+template void launch_hfb_pairfield<double>(hipStream_t, const Sp<double>*, const double*, int, double, size_t, int,
+                                           Sp<double>*);
+template void launch_hfb_pairfield<float>(hipStream_t, const Sp<float>*, const double*, int, double, size_t, int,
+                                          Sp<float>*);
 
 // ============================================================ densities
 // First derivative with the reference's boundary degradation

@@ -173,12 +173,26 @@ struct Problem {
   bool have = false;
   double tcommon = 0.0, tenergy = 0.0;  // wall time of the shared update / energies
   BcsState bcs[2];
+  // two-basis HFB state per species (M x M, column-major) and the pairing
+  // switch of a species, which an HFB collapse turns off for the run
+  struct Hfb {
+    std::vector<double> U, V, kappa;
+    double lambda = 0.0, energy = 0.0;
+  } hfb[2];
+  bool active[2] = {true, true};
+  bool hfb_on = false;
+  double* hkr = nullptr;  // kappa(r), n^3
   std::vector<ConState> cons;
   double energyDiff = 1.0;
   double massCorr = MNUC;
   Lcp* lcp = nullptr;       // the context's learned preconditioner
+  bool st_owned = false;    // st is this problem's own stream
   double lcp_stall = 0.9;   // mean |r_k| / |r_0| above which the W-block falls back
   ~Problem() {
+    if (st_owned && st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
     for (int q = 0; q < 2; ++q) {
       for (hipEvent_t e : W[q].ev) (void)hipEventDestroy(e);
       if (W[q].st) (void)hipStreamDestroy(W[q].st);
@@ -328,6 +342,7 @@ static int setup_problem(Problem& P) {
       for (int k = 0; k < c.n; ++k) S[(size_t)j * c.n + k] = nrm * std::sin(PI_ * (j + 1) * (k + 1) / (c.n + 1));
     HIPCHK(hipMemcpy(P.pS, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice));
   }
+  CHK(P.alloc(P.hkr, N3));
   CHK(P.alloc(P.mom, NHF_MAX_CON + 2));
   CHK(P.alloc(P.en, 16));
   CHK(P.alloc(P.r2b, 6));
@@ -930,6 +945,156 @@ static int broyden_step(Problem& P) {
   return NHF_OK;
 }
 
+// ------------------------------------------------------------- HFB
+// IterationData::solvePairingHFB for one species (iteration_data.cpp:
+// 605-850): Fermi-factor window [start, end], kappa(r) and Delta on the
+// device from the time-reversal-completed basis (numericalTR, kappa_field,
+// delta_matrix), bisection on lambda over the dense real H_HFB on the host.
+// `collapsed`: |Delta| < 1e-3, out = nullHFB.
+template <typename R>
+static int hfb_species_dev(Problem& P, Work& W, int q, const Sp<R>* psi, const std::vector<double>& eps,
+                           const Problem::Hfb& prev, Problem::Hfb& out, bool& collapsed) {
+  hipStream_t st = W.st;
+  const int M = P.M[q], target = P.Nq(q);
+  const size_t N3 = P.N3, ld = N3;
+  const nhf_pairing& prm = P.pairing(q);
+  const double window = prm.window, smooth = prm.window / 10.0;
+  std::vector<double> f(M);
+  for (int i = 0; i < M; ++i) {
+    double v = 1.0 / (1.0 + std::exp((eps[i] - prev.lambda - window) / smooth));
+    v *= 1.0 / (1.0 + std::exp(-(eps[i] - prev.lambda + window) / smooth));
+    f[i] = std::sqrt(v);
+  }
+  int start = 0, end = M - 1;
+  for (int i = 0; i < M; i += 2)
+    if (f[i] > P.cfg.pairing_threshold) {
+      start = i;
+      break;
+    }
+  for (int i = start; i < M; i += 2)
+    if (f[i] < P.cfg.pairing_threshold) {
+      end = i;
+      break;
+    }
+  const int nS = end - start + 1;
+  Sp<R>* ph = (Sp<R>*)W.V1;
+  Sp<R>* chi = ph + (size_t)M * ld;
+  Sp<R>* B = chi + (size_t)nS * ld;
+  launch_numerical_tr<R>(st, psi, ph, N3, ld, M);
+  const Sp<R>* phw = ph + (size_t)start * ld;
+  // chi_a = sum_b f_a f_b kappa_ab ph_b  ->  kappa(r)
+  std::vector<cd> C((size_t)nS * nS);
+  for (int a = 0; a < nS; ++a)
+    for (int b = 0; b < nS; ++b)
+      C[(size_t)a * nS + b] = f[start + a] * f[start + b] * prev.kappa[(size_t)(start + b) * M + start + a];
+  CHK(rotate_dev<R>(P, W, phw, nS, C, nS, chi, nullptr, nullptr, false));
+  launch_hfb_kappa<R>(st, phw, chi, nS, N3, ld, P.hkr);
+  // Delta_ab = V0/2 f_a f_b Re (ph_a^H B_b), a < b, antisymmetric
+  launch_hfb_pairfield<R>(st, phw, P.hkr, P.n, P.h, ld, nS, B);
+  std::vector<cd> G;
+  CHK(gram_host<R>(P, W, phw, nS, B, nS, G));
+  std::vector<double> D((size_t)nS * nS, 0.0);
+  double dn2 = 0.0;
+  for (int a = 0; a < nS; ++a)
+    for (int b = a + 1; b < nS; ++b) {
+      const double v = 0.5 * prm.V0 * f[start + a] * f[start + b] * G[(size_t)b * nS + a].real();
+      D[(size_t)b * nS + a] = v;
+      D[(size_t)a * nS + b] = -v;
+      dn2 += 2.0 * v * v;
+    }
+  collapsed = std::sqrt(dn2) < 1e-3;
+  if (collapsed) {  // nullHFB (iteration_data.cpp:591-603)
+    out = Problem::Hfb();
+    out.lambda = eps[target - 1];
+    out.U.assign((size_t)M * M, 0.0);
+    out.V.assign((size_t)M * M, 0.0);
+    for (int i = 0; i < target; ++i) out.V[(size_t)i * M + i] = 1.0;
+    return NHF_OK;
+  }
+  // bisection on lambda (findLambdaBisection) in prev +- 1.7 MeV for
+  // sum_k |V_k|^2 + start = N, tolerance 1e-12, at most 100 steps
+  double lmin = prev.lambda - 1.7, lmax = prev.lambda + 1.7, cur = prev.lambda;
+  const int n2 = 2 * nS;
+  std::vector<double> H((size_t)n2 * n2), w(n2), Z((size_t)n2 * n2), U((size_t)nS * nS), V((size_t)nS * nS);
+  for (int it = 0; it < 100; ++it) {
+    std::fill(H.begin(), H.end(), 0.0);
+    for (int i = 0; i < nS; ++i) {
+      H[(size_t)i * n2 + i] = eps[start + i] - cur;
+      H[(size_t)(nS + i) * n2 + nS + i] = -(eps[start + i] - cur);
+    }
+    for (int j = 0; j < nS; ++j)
+      for (int i = 0; i < nS; ++i) {
+        H[(size_t)(nS + j) * n2 + i] = D[(size_t)j * nS + i];
+        H[(size_t)j * n2 + nS + i] = -D[(size_t)j * nS + i];
+      }
+    if (!eigh_real_sym(n2, H.data(), w.data(), Z.data())) return NHF_ESOLVER;
+    double num = start;
+    for (int k = 0; k < nS; ++k) {
+      const double* Wk = &Z[(size_t)(k + nS) * n2];
+      for (int i = 0; i < nS; ++i) {
+        U[(size_t)k * nS + i] = Wk[i];
+        V[(size_t)k * nS + i] = Wk[nS + i];
+        num += Wk[nS + i] * Wk[nS + i];
+      }
+    }
+    const double diff = num - target;
+    if (std::abs(diff) < 1e-12) break;
+    if (diff > 0)
+      lmax = cur;
+    else
+      lmin = cur;
+    cur = 0.5 * (lmin + lmax);
+  }
+  out = Problem::Hfb();
+  out.lambda = cur;
+  double ep = 0.0;  // tr(Delta kappa^T) / 2, kappa = V U^T
+  for (int a = 0; a < nS; ++a)
+    for (int b = 0; b < nS; ++b) {
+      double k = 0.0;
+      for (int c = 0; c < nS; ++c) k += V[(size_t)c * nS + a] * U[(size_t)c * nS + b];
+      ep += D[(size_t)b * nS + a] * k;
+    }
+  out.energy = 0.5 * ep;
+  out.U.assign((size_t)M * M, 0.0);
+  out.V.assign((size_t)M * M, 0.0);
+  for (int c = 0; c < nS; ++c)
+    for (int i = 0; i < nS; ++i) {
+      out.U[(size_t)(start + c) * M + start + i] = U[(size_t)c * nS + i];
+      out.V[(size_t)(start + c) * M + start + i] = V[(size_t)c * nS + i];
+    }
+  for (int i = 0; i < start; ++i) out.V[(size_t)i * M + i] = 1.0;
+  for (int i = end + 1; i < M; ++i) out.U[(size_t)i * M + i] = 1.0;
+  out.kappa.assign((size_t)M * M, 0.0);  // U V^T of the full basis
+  for (int b = 0; b < M; ++b)
+    for (int a = 0; a < M; ++a) {
+      double k = 0.0;
+      for (int c = 0; c < M; ++c) k += out.U[(size_t)c * M + a] * out.V[(size_t)c * M + b];
+      out.kappa[(size_t)b * M + a] = k;
+    }
+  return NHF_OK;
+}
+
+// canonical states phi W diag(sqrt v2), V V^T = W diag(v2) W^T, v2 < 1e-12
+// set to zero (neutronsFromPairing, iteration_data.cpp:320-338)
+template <typename R>
+static int hfb_canonical_dev(Problem& P, Work& W, int q, const Sp<R>* psi, const Problem::Hfb& h, Sp<R>* out) {
+  const int M = P.M[q];
+  std::vector<double> rho((size_t)M * M), w(M), Wv((size_t)M * M);
+  for (int b = 0; b < M; ++b)
+    for (int a = 0; a < M; ++a) {
+      double s = 0.0;
+      for (int c = 0; c < M; ++c) s += h.V[(size_t)c * M + a] * h.V[(size_t)c * M + b];
+      rho[(size_t)b * M + a] = s;
+    }
+  if (!eigh_real_sym(M, rho.data(), w.data(), Wv.data())) return NHF_ESOLVER;
+  std::vector<cd> C((size_t)M * M);
+  for (int c = 0; c < M; ++c) {
+    const double sv = std::sqrt(w[c] < 1e-12 ? 0.0 : w[c]);
+    for (int l = 0; l < M; ++l) C[(size_t)c * M + l] = Wv[(size_t)c * M + l] * sv;
+  }
+  return rotate_dev<R>(P, W, psi, M, C, M, out, nullptr, nullptr, false);
+}
+
 // IterationData::updateQuantities (iteration_data.cpp:852-1125), common
 // part: occupations (solvePairingBCS, :384-465), densities with linear
 // mixing, density moments and the constraint field coefficients.  The
@@ -952,38 +1117,84 @@ static int update_common(Problem& P, Sp<R>* const psi[2], const std::vector<doub
     nul[q].qp.resize(M);
     for (int i = 0; i < M; ++i) nul[q].qp[i] = nul[q].v2[i] * eps[q][i];
   }
-  if (c.pairing_type == NHF_PAIR_NONE) {
-    P.bcs[0] = nul[0];
-    P.bcs[1] = nul[1];
-  } else {
-    for (int q = 0; q < 2; ++q) {
-      if (!P.pairing(q).active) {
-        P.bcs[q] = nul[q];
-        continue;
+  // density sources: the states with occupation weights v^2, or for HFB
+  // the canonical states with weight 1
+  const Sp<R>* src[2] = {psi[0], psi[1]};
+  std::vector<double> occ[2];
+  if (c.pairing_type == NHF_PAIR_HFB && iter >= c.start_hfb_iter) {
+    // updateQuantities (iteration_data.cpp:871-921): HFB from startHFBIter,
+    // its pairing tensor seeded from the last BCS solution
+    P.hfb_on = true;
+    if (iter == c.start_hfb_iter)
+      for (int q = 0; q < 2; ++q) {
+        const int M = P.M[q];
+        P.hfb[q] = Problem::Hfb();
+        P.hfb[q].kappa.assign((size_t)M * M, 0.0);
+        P.hfb[q].lambda = P.bcs[q].lambda;
+        for (int i = 0; i + 1 < M; i += 2) {
+          const double uv = std::sqrt(P.bcs[q].u2[i] * P.bcs[q].v2[i]);
+          P.hfb[q].kappa[(size_t)(i + 1) * M + i] = uv;
+          P.hfb[q].kappa[(size_t)i * M + i + 1] = -uv;
+        }
       }
-      BcsState b;
-      CHK(bcs_species<R>(P, P.W[q], q, psi[q], eps[q], b));
-      P.bcs[q] = b;
+    for (int q = 0; q < 2; ++q) {
+      if (P.active[q]) {
+        Problem::Hfb r;
+        bool collapsed = false;
+        CHK(hfb_species_dev<R>(P, P.W[q], q, psi[q], eps[q], P.hfb[q], r, collapsed));
+        if (std::isnan(r.energy)) {
+          // keep the previous result (iteration_data.cpp:821-846)
+        } else if (std::abs(r.energy) < 1e-12) {
+          P.active[q] = false;  // collapsed pairing: off for the rest of the run
+        } else {
+          P.hfb[q] = r;
+        }
+      }
+      if (P.active[q]) {
+        Sp<R>* can = (Sp<R>*)P.W[q].V;
+        CHK(hfb_canonical_dev<R>(P, P.W[q], q, psi[q], P.hfb[q], can));
+        src[q] = can;
+        occ[q].assign(P.M[q], 1.0);
+      } else {
+        occ[q] = nul[q].v2;
+      }
     }
-    double sn = 0, sp = 0;
-    for (double v : P.bcs[0].v2) sn += v;
-    for (double v : P.bcs[1].v2) sp += v;
-    if (std::isnan(sn) || std::isnan(sp)) {
+  } else {
+    if (c.pairing_type == NHF_PAIR_NONE) {
       P.bcs[0] = nul[0];
       P.bcs[1] = nul[1];
+    } else {
+      for (int q = 0; q < 2; ++q) {
+        if (!P.active[q]) {
+          P.bcs[q] = nul[q];
+          continue;
+        }
+        BcsState b;
+        CHK(bcs_species<R>(P, P.W[q], q, psi[q], eps[q], b));
+        P.bcs[q] = b;
+      }
+      double sn = 0, sp = 0;
+      for (double v : P.bcs[0].v2) sn += v;
+      for (double v : P.bcs[1].v2) sp += v;
+      if (std::isnan(sn) || std::isnan(sp)) {
+        P.bcs[0] = nul[0];
+        P.bcs[1] = nul[1];
+      }
     }
+    occ[0] = P.bcs[0].v2;
+    occ[1] = P.bcs[1].v2;
   }
   for (int q = 0; q < 2; ++q)
-    HIPCHK(hipMemcpyAsync(P.occ[q], P.bcs[q].v2.data(), sizeof(double) * P.M[q], hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(P.occ[q], occ[q].data(), sizeof(double) * P.M[q], hipMemcpyHostToDevice, st));
   const int first = P.have ? 0 : 1;
   if (c.mixing == NHF_MIX_BROYDEN && P.have) {
     for (int q = 0; q < 2; ++q)
-      launch_densities<R>(st, psi[q], P.N3, P.M[q], P.occ[q], P.n, P.h, 1.0, 1, P.rho_o[q], P.tau_o[q], P.J9_o[q],
+      launch_densities<R>(st, src[q], P.N3, P.M[q], P.occ[q], P.n, P.h, 1.0, 1, P.rho_o[q], P.tau_o[q], P.J9_o[q],
                           P.divJ_o[q]);
     CHK(broyden_step(P));
   } else {
     for (int q = 0; q < 2; ++q)
-      launch_densities<R>(st, psi[q], P.N3, P.M[q], P.occ[q], P.n, P.h, c.density_mix, first, P.rho[q], P.tau[q],
+      launch_densities<R>(st, src[q], P.N3, P.M[q], P.occ[q], P.n, P.h, c.density_mix, first, P.rho[q], P.tau[q],
                           P.J9[q], P.divJ[q]);
   }
   int kinds[NHF_MAX_CON];
@@ -1048,7 +1259,10 @@ static int energies_dev(Problem& P, double* T) {
   std::fill(T, T + NHF_NTERMS, 0.0);
   for (int t = 0; t < 16; ++t) T[t] = e[t];
   auto ep = [&](int q) {
-    return (P.cfg.pairing_type == NHF_PAIR_BCS && P.pairing(q).active) ? P.bcs[q].Epair : 0.0;
+    if (!P.active[q] || P.cfg.pairing_type == NHF_PAIR_NONE) return 0.0;
+    // HFB: the seed steps report the BCS pairing energy (the reference reads
+    // an HFB energy no step has set yet there, iteration_data.cpp:31-33)
+    return (P.cfg.pairing_type == NHF_PAIR_HFB && P.hfb_on) ? P.hfb[q].energy : P.bcs[q].Epair;
   };
   T[NHF_E_PAIR_N] = ep(0);
   T[NHF_E_PAIR_P] = ep(1);
@@ -1257,8 +1471,8 @@ static int scf_dev(Problem& P, std::vector<double> eps[2], nhf_result* res, doub
   const double A = c.A, R0 = 1.2 * std::cbrt(A);
   res->beta2 = 4 * PI_ * o[4] / (3 * A * R0 * R0);
   res->gamma = std::atan2(std::sqrt(2.0) * o[5], o[4]);
-  res->lambda_n = P.bcs[0].lambda;
-  res->lambda_p = P.bcs[1].lambda;
+  res->lambda_n = P.hfb_on ? P.hfb[0].lambda : P.bcs[0].lambda;
+  res->lambda_p = P.hfb_on ? P.hfb[1].lambda : P.bcs[1].lambda;
   // the final states live in buffer `cur`
   if (cur != 0)
     for (int q = 0; q < 2; ++q) std::swap(P.psi[q][0], P.psi[q][1]);
@@ -1353,11 +1567,13 @@ int nhf_problem(void* ctx, const nhf_config* cfg, int* pid) {
       cfg->M_n < 1 || cfg->M_p < 1 || cfg->M_n < cfg->A - cfg->Z || cfg->M_p < cfg->Z || cfg->n_con < 0 ||
       cfg->n_con > NHF_MAX_CON || (cfg->precision != NHF_PREC_FP64 && cfg->precision != NHF_PREC_MIXED))
     return NHF_EINVAL;
-  if (cfg->pairing_type == NHF_PAIR_BCS) {
+  if (cfg->pairing_type == NHF_PAIR_BCS || cfg->pairing_type == NHF_PAIR_HFB) {
     if ((cfg->pair_n.active && cfg->M_n <= cfg->A - cfg->Z) || (cfg->pair_p.active && cfg->M_p <= cfg->Z))
       return NHF_EINVAL;
+    // HFB is seeded from a BCS solution: at least one BCS step first
+    if (cfg->pairing_type == NHF_PAIR_HFB && cfg->start_hfb_iter < 1) return NHF_EINVAL;
   } else if (cfg->pairing_type != NHF_PAIR_NONE) {
-    return NHF_EINVAL;  // HFB is not available in this build
+    return NHF_EINVAL;
   }
   if ((cfg->mixing != NHF_MIX_LINEAR && cfg->mixing != NHF_MIX_BROYDEN) || cfg->mix_hist < 0 ||
       cfg->mix_hist > NHF_MAX_HIST || cfg->mix_alpha < 0 || (cfg->wstep != NHF_W_JACOBI_CG && cfg->wstep != NHF_W_LEARNED) ||
@@ -1367,7 +1583,10 @@ int nhf_problem(void* ctx, const nhf_config* cfg, int* pid) {
   HIPCHK(hipSetDevice(c->device));
   auto P = std::make_unique<Problem>();
   P->cfg = *cfg;
-  P->st = c->st;
+  // every problem has its own streams, so that problems can run
+  // concurrently (nhf_batch_run)
+  HIPCHK(hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking));
+  P->st_owned = true;
   P->lcp = &c->lcp;
   int rc = setup_problem(*P);
   if (rc != NHF_OK) return rc;
@@ -1503,14 +1722,56 @@ int nhf_gcg_step(void* ctx, int pid, int q, const double* conjdir, int p_cols, c
   return NHF_OK;
 }
 
-int nhf_scf_run(void* ctx, int pid, double* psi_n, double* psi_p, double* eps_n, double* eps_p, nhf_result* res,
-                double* trace, int trace_cap) {
+int nhf_ws_guess(void* ctx, int pid, double beta2, double beta3, const nhf_ws_params* wsp, const nhf_gcg_params* gcg,
+                 double* psi_n, double* psi_p, double* eps_n, double* eps_p) {
   Problem* P = get_problem(ctx, pid);
-  if (!P || !psi_n || !psi_p || !eps_n || !eps_p || !res) return NHF_EINVAL;
-  CHK(upload_states(*P, psi_n, P->psi[0][0], P->M[0]));
-  CHK(upload_states(*P, psi_p, P->psi[1][0], P->M[1]));
-  // a run starts from scratch: no mixing history, Coulomb warm start zero
+  if (!P || !psi_n || !psi_p || !eps_n || !eps_p) return NHF_EINVAL;
+  const nhf_gcg_params prm = gcg ? *gcg : nhf_gcg_params{10, 40, 1e-2, 1e-4};
+  const nhf_config& c = P->cfg;
+  const size_t N3 = P->N3, rows = 2 * N3;
+  std::vector<double> f(N3), gf(3 * N3), U(N3), Wf(3 * N3);
+  CHK(nhf_ws_fields(c.n, c.box_size, c.A, c.Z, beta2, beta3, c.spin_orbit, wsp, f.data(), gf.data(), U.data(),
+                    Wf.data()));
+  // one solve with the larger state count; both species take its lowest
+  // states (main.cpp:98-110)
+  const int q = P->M[0] >= P->M[1] ? 0 : 1, m = P->M[q];
+  std::vector<std::complex<double>> X(rows * m);
+  CHK(nhf_ho_guess(c.n, c.box_size, m, std::pow((double)c.A, 0.3333333) * 1.20, (double*)X.data()));
+  CHK(nhf_set_fields(ctx, pid, q, f.data(), gf.data(), U.data(), Wf.data()));
+  std::vector<double> eps(m);
+  CHK(nhf_gcg_step(ctx, pid, q, nullptr, 0, &prm, (double*)X.data(), m, eps.data(), P->prec));
+  // trapezoid normalisation (wavefunction.cpp:362-370)
+  const int n = c.n;
+  std::vector<double> w1(n, 1.0);
+  w1[0] = w1[n - 1] = 0.5;
+  for (int col = 0; col < m; ++col) {
+    const std::complex<double>* x = X.data() + (size_t)col * rows;
+    double s = 0.0;
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          const size_t p = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k);
+          s += w1[i] * w1[j] * w1[k] * (std::norm(x[2 * p]) + std::norm(x[2 * p + 1]));
+        }
+    const double inv = 1.0 / std::sqrt(s * P->h * P->h * P->h);
+    for (int sp = 0; sp < 2; ++sp) {
+      if (col >= P->M[sp]) continue;
+      std::complex<double>* out = (std::complex<double>*)(sp ? psi_p : psi_n) + (size_t)col * rows;
+      for (size_t r = 0; r < rows; ++r) out[r] = x[r] * inv;
+      (sp ? eps_p : eps_n)[col] = eps[col];
+    }
+  }
+  return NHF_OK;
+}
+
+// A run starts from scratch: no mixing or Broyden history, no pairing or
+// constraint state, Coulomb warm start zero.
+static int reset_run_state(Problem& Pr) {
+  Problem* P = &Pr;
   P->have = false;
+  P->hfb_on = false;
+  P->active[0] = P->cfg.pair_n.active != 0;
+  P->active[1] = P->cfg.pair_p.active != 0;
   P->bro.next = 0;
   P->bro.valid = 0;
   P->bro.have_prev = false;
@@ -1522,6 +1783,16 @@ int nhf_scf_run(void* ctx, int pid, double* psi_n, double* psi_p, double* eps_n,
     P->cons.push_back(k);
   }
   HIPCHK(hipMemsetAsync(P->UCdir, 0, sizeof(double) * P->N3, P->st));
+  return NHF_OK;
+}
+
+int nhf_scf_run(void* ctx, int pid, double* psi_n, double* psi_p, double* eps_n, double* eps_p, nhf_result* res,
+                double* trace, int trace_cap) {
+  Problem* P = get_problem(ctx, pid);
+  if (!P || !psi_n || !psi_p || !eps_n || !eps_p || !res) return NHF_EINVAL;
+  CHK(upload_states(*P, psi_n, P->psi[0][0], P->M[0]));
+  CHK(upload_states(*P, psi_p, P->psi[1][0], P->M[1]));
+  CHK(reset_run_state(*P));
   std::vector<double> eps[2];
   eps[0].assign(eps_n, eps_n + P->M[0]);
   eps[1].assign(eps_p, eps_p + P->M[1]);
@@ -1534,6 +1805,59 @@ int nhf_scf_run(void* ctx, int pid, double* psi_n, double* psi_p, double* eps_n,
   CHK(download_states(*P, P->psi[1][0], psi_p, P->M[1]));
   std::copy(eps[0].begin(), eps[0].end(), eps_n);
   std::copy(eps[1].begin(), eps[1].end(), eps_p);
+  return NHF_OK;
+}
+
+int nhf_curve_run(void* ctx, int pid, const double* mu20, int npts, double* psi_n, double* psi_p, double* eps_n,
+                  double* eps_p, nhf_result* res) {
+  Problem* P = get_problem(ctx, pid);
+  if (!P || !mu20 || npts < 1 || !psi_n || !psi_p || !eps_n || !eps_p || !res) return NHF_EINVAL;
+  if (P->cfg.n_con < 1 || P->cfg.con[P->cfg.n_con - 1].kind != NHF_CON_Q20) return NHF_EINVAL;
+  CHK(upload_states(*P, psi_n, P->psi[0][0], P->M[0]));
+  CHK(upload_states(*P, psi_p, P->psi[1][0], P->M[1]));
+  CHK(reset_run_state(*P));
+  std::vector<double> eps[2];
+  eps[0].assign(eps_n, eps_n + P->M[0]);
+  eps[1].assign(eps_p, eps_p + P->M[1]);
+  for (int i = 0; i < npts; ++i) {
+    // the quadrupole constraint is the last one; its multiplier and the
+    // mixing / pairing state carry over (main.cpp:186-189)
+    P->cons.back().c.target = mu20[i];
+    std::memset(res + i, 0, sizeof(res[i]));
+    if (P->prec == NHF_PREC_FP64)
+      CHK(scf_dev<double>(*P, eps, res + i, nullptr, 0));
+    else
+      CHK(scf_dev<float>(*P, eps, res + i, nullptr, 0));
+  }
+  CHK(download_states(*P, P->psi[0][0], psi_n, P->M[0]));
+  CHK(download_states(*P, P->psi[1][0], psi_p, P->M[1]));
+  std::copy(eps[0].begin(), eps[0].end(), eps_n);
+  std::copy(eps[1].begin(), eps[1].end(), eps_p);
+  return NHF_OK;
+}
+
+int nhf_batch_run(void* ctx, nhf_batch_item* items, int count) {
+  if (!ctx || !items || count < 1) return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  for (int i = 0; i < count; ++i) {
+    if (!get_problem(ctx, items[i].pid)) return NHF_EINVAL;
+    for (int j = 0; j < i; ++j)
+      if (items[j].pid == items[i].pid) return NHF_EINVAL;  // a problem's buffers serve one run at a time
+  }
+  std::vector<std::thread> th;
+  th.reserve(count);
+  for (int i = 0; i < count; ++i)
+    th.emplace_back([c, ctx, &items, i] {
+      nhf_batch_item& it = items[i];
+      if (hipSetDevice(c->device) != hipSuccess) {
+        it.rc = NHF_EHIP;
+        return;
+      }
+      it.rc = nhf_scf_run(ctx, it.pid, it.psi_n, it.psi_p, it.eps_n, it.eps_p, it.res, it.trace, it.trace_cap);
+    });
+  for (auto& t : th) t.join();
+  for (int i = 0; i < count; ++i)
+    if (items[i].rc != NHF_OK) return items[i].rc;
   return NHF_OK;
 }
 

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .config import NhfConfig, NhfGcgParams, NhfResult, PREC_FP64
+from .config import NhfBatchItem, NhfConfig, NhfGcgParams, NhfResult, NhfWsParams, PREC_FP64
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnhf_hip.so")
@@ -42,11 +42,18 @@ def lib():
                                    ctypes.POINTER(NhfGcgParams), _D, ctypes.c_int, _D, ctypes.c_int]
         L.nhf_scf_run.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, ctypes.POINTER(NhfResult), _D,
                                   ctypes.c_int]
+        L.nhf_curve_run.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D, _D, _D, _D,
+                                    ctypes.POINTER(NhfResult)]
+        L.nhf_batch_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(NhfBatchItem), ctypes.c_int]
         L.nhf_bench_hpsi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _D]
         L.nhf_kernel_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         L.nhf_phase_times.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         L.nhf_ho_guess.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]
+        L.nhf_ws_fields.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, ctypes.POINTER(NhfWsParams), _D, _D, _D, _D]
+        L.nhf_ws_guess.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   ctypes.POINTER(NhfWsParams), ctypes.POINTER(NhfGcgParams), _D, _D, _D, _D]
         L.nhf_eigh_lower.argtypes = [ctypes.c_int, _D, _D, _D]
         L.nhf_block_gram.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, _D, ctypes.c_int, _D,
                                      ctypes.c_int, ctypes.c_int, _D]
@@ -141,7 +148,8 @@ def trapezoid_normalize(psi, n, box_size):
 
 
 class Context:
-    """One HIP device, one stream (nhf_create / nhf_destroy)."""
+    """One HIP device (nhf_create / nhf_destroy); its problems each own
+    their streams."""
 
     def __init__(self, device=0):
         h = ctypes.c_void_p()
@@ -169,6 +177,28 @@ class Context:
         pid = ctypes.c_int()
         _chk(lib().nhf_problem(self.h, ctypes.byref(cfg), ctypes.byref(pid)))
         return Problem(self, pid.value, cfg)
+
+    def batch_run(self, runs, trace_cap=4000):
+        """Independent SCF runs concurrently on the device (nhf_batch_run).
+        runs: [(problem, (psi_n, psi_p, eps_n, eps_p)), ...] with distinct
+        problems; returns per run what Problem.scf_run returns."""
+        items = (NhfBatchItem * len(runs))()
+        keep = []
+        for it, (prob, (pn, pp, en, ep)) in zip(items, runs):
+            pn = _cols(pn).copy(order="F")
+            pp = _cols(pp).copy(order="F")
+            en = np.array(en, dtype=np.float64)
+            ep = np.array(ep, dtype=np.float64)
+            res = NhfResult()
+            tr = np.zeros((trace_cap, 4))
+            keep.append((pn, pp, en, ep, res, tr))
+            it.pid = prob.pid
+            it.psi_n, it.psi_p, it.eps_n, it.eps_p, it.trace = _dp(pn), _dp(pp), _dp(en), _dp(ep), _dp(tr)
+            it.res = ctypes.pointer(res)
+            it.trace_cap = trace_cap
+        _chk(lib().nhf_batch_run(self.h, items, len(runs)))
+        return [(pn, pp, en, ep, res, tr[: min(res.iterations, trace_cap)].copy())
+                for pn, pp, en, ep, res, tr in keep]
 
     def block_gram(self, A, B=None, prec=PREC_FP64):
         """G = A^H B with the library's device block product (A, B: complex
@@ -237,6 +267,16 @@ class Context:
         return a.value, b.value
 
 
+def ws_fields(n, box_size, A, Z, beta2=0.0, beta3=0.0, spin_orbit=True, params=None):
+    """Fields (f, grad f, U, W) of the deformed Woods-Saxon start-up
+    Hamiltonian (nhf_ws_fields); params None = the reference's file values."""
+    N3 = n ** 3
+    f, U, gf, W = np.zeros(N3), np.zeros(N3), np.zeros(3 * N3), np.zeros(3 * N3)
+    _chk(lib().nhf_ws_fields(n, box_size, A, Z, beta2, beta3, int(spin_orbit),
+                             None if params is None else ctypes.byref(params), _dp(f), _dp(gf), _dp(U), _dp(W)))
+    return f, gf, U, W
+
+
 class Problem:
     def __init__(self, ctx, pid, cfg):
         self.ctx, self.pid, self.cfg = ctx, pid, cfg
@@ -283,6 +323,32 @@ class Problem:
         _chk(lib().nhf_gcg_step(self.ctx.h, self.pid, q, _dp(cj), pc, ctypes.byref(params), _dp(psi), m, _dp(eps),
                                 self.cfg.precision if prec is None else prec))
         return psi, eps
+
+    def ws_guess(self, beta2=0.0, beta3=0.0, params=None, gcg=None):
+        """Woods-Saxon start-up states (nhf_ws_guess): (psi_n, psi_p, eps_n,
+        eps_p), ready for scf_run."""
+        rows = 2 * self.N3
+        psi_n = np.zeros((rows, self.cfg.M_n), dtype=np.complex128, order="F")
+        psi_p = np.zeros((rows, self.cfg.M_p), dtype=np.complex128, order="F")
+        eps_n, eps_p = np.zeros(self.cfg.M_n), np.zeros(self.cfg.M_p)
+        _chk(lib().nhf_ws_guess(self.ctx.h, self.pid, beta2, beta3, None if params is None else ctypes.byref(params),
+                                None if gcg is None else ctypes.byref(gcg), _dp(psi_n), _dp(psi_p), _dp(eps_n),
+                                _dp(eps_p)))
+        return psi_n, psi_p, eps_n, eps_p
+
+    def curve_run(self, mu20, psi_n, psi_p, eps_n, eps_p):
+        """Deformation curve (nhf_curve_run): one result per mu20 target;
+        returns (psi_n, psi_p, eps_n, eps_p) of the last point and the
+        results."""
+        mu = np.ascontiguousarray(mu20, dtype=np.float64)
+        psi_n = _cols(psi_n).copy(order="F")
+        psi_p = _cols(psi_p).copy(order="F")
+        eps_n = np.array(eps_n, dtype=np.float64)
+        eps_p = np.array(eps_p, dtype=np.float64)
+        res = (NhfResult * len(mu))()
+        _chk(lib().nhf_curve_run(self.ctx.h, self.pid, _dp(mu), len(mu), _dp(psi_n), _dp(psi_p), _dp(eps_n),
+                                 _dp(eps_p), res))
+        return (psi_n, psi_p, eps_n, eps_p), list(res)
 
     def bench_hpsi(self, q, m, prec=None, reps=50):
         ms = ctypes.c_double()

@@ -1083,6 +1083,205 @@ static BCSRes bcs_iter(const Grid& g, const vecc& psi, const vecd& eps, int M, i
   return R;
 }
 
+// ------------------------------------------------------------------ HFB
+// IterationData::solvePairingHFB (iteration_data.cpp:605-850), two-basis
+// HFB in the window of HF states selected by the Fermi factor, with
+// numericalTR (:580-589), kappa_field (:467-501), delta_matrix (:503-538),
+// findLambdaBisection (:555-563) and nullHFB (:591-603).  Matrices M x M,
+// column-major.
+struct HFBRes {
+  vecd U, V, kappa;
+  double lambda = 0.0, energy = 0.0;
+};
+// odd columns replaced by the time reverse of the preceding even column
+static void numerical_tr(size_t N3, const vecc& psi, int M, vecc& out) {
+  const size_t nr = 2 * N3;
+  out = psi;
+  for (int c = 1; c < M; c += 2) time_reverse(N3, &psi[(size_t)(c - 1) * nr], &out[(size_t)c * nr]);
+}
+static HFBRes null_hfb(int target, const vecd& eps) {
+  const int M = (int)eps.size();
+  HFBRes r;
+  r.lambda = eps[target - 1];
+  r.U.assign((size_t)M * M, 0.0);
+  r.V.assign((size_t)M * M, 0.0);
+  for (int i = 0; i < target; ++i) r.V[(size_t)i * M + i] = 1.0;
+  r.energy = 0.0;
+  return r;
+}
+// pairing window of the Fermi factors (iteration_data.cpp:637-660)
+static void hfb_window(const vecd& f, double threshold, int& start, int& end) {
+  const int M = (int)f.size();
+  start = 0;
+  end = M - 1;
+  for (int i = 0; i < M; i += 2)
+    if (f[i] > threshold) {
+      start = i;
+      break;
+    }
+  for (int i = start; i < M; i += 2)
+    if (f[i] < threshold) {
+      end = i;
+      break;
+    }
+}
+// kappa(r) and Delta of the window [start, end] from the time-reversed
+// basis ph (kappa_field, delta_matrix):
+//   kappa(r) = Re sum_ab sum_s -sigma_s ph_b(r, -s) ph_a(r, s) f_a f_b kappa_ab
+//   Delta_ab = V0/2 Re int conj(sum_s sigma_s ph_a(s) ph_b(-s) f_a f_b) kappa(r)
+// (trapezoid), antisymmetric (a < b computed, Delta_ba = -Delta_ab).
+static void hfb_fields(const Grid& g, const vecc& ph, int M, const vecd& f, const vecd& kappa, int start, int end,
+                       double V0, vecd& kr, vecd& D) {
+  const size_t N3 = g.N3, nr = 2 * N3;
+  const int nS = end - start + 1;
+  auto up = [&](int a, size_t p) { return ph[(size_t)a * nr + 2 * p]; };
+  auto dn = [&](int a, size_t p) { return ph[(size_t)a * nr + 2 * p + 1]; };
+  kr.assign(N3, 0.0);
+#pragma omp parallel for
+  for (size_t p = 0; p < N3; ++p) {
+    cd acc = 0.0;
+    for (int a = start; a <= end; ++a)
+      for (int b = start; b <= end; ++b) {
+        const double k = kappa[(size_t)b * M + a] * f[a] * f[b];
+        if (k == 0.0) continue;
+        acc += -(up(a, p) * dn(b, p) - dn(a, p) * up(b, p)) * k;
+      }
+    kr[p] = acc.real();
+  }
+  D.assign((size_t)nS * nS, 0.0);
+#pragma omp parallel for collapse(2)
+  for (int a = start; a <= end; ++a)
+    for (int b = start; b <= end; ++b) {
+      if (b <= a) continue;
+      cd val = 0.0;
+      for (int k = 0; k < g.n; ++k)
+        for (int j = 0; j < g.n; ++j)
+          for (int i = 0; i < g.n; ++i) {
+            const size_t p = g.idx(i, j, k);
+            const cd t = (up(a, p) * dn(b, p) - dn(a, p) * up(b, p)) * (f[a] * f[b]);
+            val += std::conj(t) * kr[p] * (g.h * g.h * g.h * g.w(i, j, k));
+          }
+      D[(size_t)(b - start) * nS + (a - start)] = 0.5 * V0 * val.real();
+      D[(size_t)(a - start) * nS + (b - start)] = -0.5 * V0 * val.real();
+    }
+}
+// bisection on lambda in lambda0 +- 1.7 MeV for sum_k |V_k|^2 + start =
+// target (tolerance 1e-12, at most 100 steps), with H_HFB =
+// [[diag(e - l), D], [-D, -diag(e - l)]]; U_k, V_k = upper and lower halves of
+// the eigenvectors of the nS positive quasiparticle energies.  Returns the
+// pairing energy tr(D kappa^T) / 2 with kappa = V U^T.
+static double hfb_solve(int nS, const double* e, const vecd& D, double lambda0, double target, int start, vecd& U,
+                        vecd& V, double& lambda) {
+  double lmin = lambda0 - 1.7, lmax = lambda0 + 1.7, cur = lambda0;
+  const int n2 = 2 * nS;
+  vecd H((size_t)n2 * n2), w(n2), Z((size_t)n2 * n2);
+  U.assign((size_t)nS * nS, 0.0);
+  V.assign((size_t)nS * nS, 0.0);
+  for (int it = 0; it < 100; ++it) {
+    std::fill(H.begin(), H.end(), 0.0);
+    for (int i = 0; i < nS; ++i) {
+      H[(size_t)i * n2 + i] = e[i] - cur;
+      H[(size_t)(nS + i) * n2 + nS + i] = -(e[i] - cur);
+    }
+    for (int j = 0; j < nS; ++j)
+      for (int i = 0; i < nS; ++i) {
+        H[(size_t)(nS + j) * n2 + i] = D[(size_t)j * nS + i];  // top right: D
+        H[(size_t)j * n2 + nS + i] = -D[(size_t)j * nS + i];   // bottom left: -D
+      }
+    eigh_real(n2, H.data(), w.data(), Z.data());
+    double num = start;
+    for (int k = 0; k < nS; ++k) {
+      const double* W = &Z[(size_t)(k + nS) * n2];
+      for (int i = 0; i < nS; ++i) {
+        U[(size_t)k * nS + i] = W[i];
+        V[(size_t)k * nS + i] = W[nS + i];
+        num += W[nS + i] * W[nS + i];
+      }
+    }
+    const double diff = num - target;
+    if (std::abs(diff) < 1e-12) break;
+    if (diff > 0)
+      lmax = cur;
+    else
+      lmin = cur;
+    cur = 0.5 * (lmin + lmax);
+  }
+  lambda = cur;
+  double ep = 0.0;
+  for (int a = 0; a < nS; ++a)
+    for (int b = 0; b < nS; ++b) {
+      double k = 0.0;
+      for (int c = 0; c < nS; ++c) k += V[(size_t)c * nS + a] * U[(size_t)c * nS + b];
+      ep += D[(size_t)b * nS + a] * k;
+    }
+  return 0.5 * ep;
+}
+static HFBRes hfb_species(const Grid& g, const vecc& phi, const vecd& eps, int M, const HFBRes& prev, int target,
+                          const nhf_pairing& prm, double threshold) {
+  const double window = prm.window, smooth = prm.window / 10.0;
+  vecd f(M);
+  for (int i = 0; i < M; ++i) {
+    double v = 1.0 / (1.0 + std::exp((eps[i] - prev.lambda - window) / smooth));
+    v *= 1.0 / (1.0 + std::exp(-(eps[i] - prev.lambda + window) / smooth));
+    f[i] = std::sqrt(v);
+  }
+  int start, end;
+  hfb_window(f, threshold, start, end);
+  const int nS = end - start + 1;
+  vecc ph;
+  numerical_tr(g.N3, phi, M, ph);
+  vecd kr, D;
+  hfb_fields(g, ph, M, f, prev.kappa, start, end, prm.V0, kr, D);
+  double dn2 = 0.0;
+  for (double v : D) dn2 += v * v;
+  if (std::sqrt(dn2) < 1e-3) return null_hfb(target, eps);
+  vecd U, V;
+  HFBRes r;
+  r.energy = hfb_solve(nS, &eps[start], D, prev.lambda, target, start, U, V, r.lambda);
+  r.U.assign((size_t)M * M, 0.0);
+  r.V.assign((size_t)M * M, 0.0);
+  for (int c = 0; c < nS; ++c)
+    for (int i = 0; i < nS; ++i) {
+      r.U[(size_t)(start + c) * M + start + i] = U[(size_t)c * nS + i];
+      r.V[(size_t)(start + c) * M + start + i] = V[(size_t)c * nS + i];
+    }
+  for (int i = 0; i < start; ++i) r.V[(size_t)i * M + i] = 1.0;
+  for (int i = end + 1; i < M; ++i) r.U[(size_t)i * M + i] = 1.0;
+  r.kappa.assign((size_t)M * M, 0.0);  // U V^T of the full basis
+  for (int b = 0; b < M; ++b)
+    for (int a = 0; a < M; ++a) {
+      double k = 0.0;
+      for (int c = 0; c < M; ++c) k += r.U[(size_t)c * M + a] * r.V[(size_t)c * M + b];
+      r.kappa[(size_t)b * M + a] = k;
+    }
+  return r;
+}
+// canonical states: phi W diag(sqrt v2) with V V^T = W diag(v2) W^T
+// (neutronsFromPairing, iteration_data.cpp:320-338)
+static void hfb_canonical(size_t N3, const vecc& phi, int M, const HFBRes& r, vecc& out) {
+  vecd rho((size_t)M * M), w(M), W((size_t)M * M);
+  for (int b = 0; b < M; ++b)
+    for (int a = 0; a < M; ++a) {
+      double s = 0.0;
+      for (int c = 0; c < M; ++c) s += r.V[(size_t)c * M + a] * r.V[(size_t)c * M + b];
+      rho[(size_t)b * M + a] = s;
+    }
+  eigh_real(M, rho.data(), w.data(), W.data());
+  const size_t nr = 2 * N3;
+  out.assign((size_t)M * nr, 0.0);
+  for (int c = 0; c < M; ++c) {
+    const double s = std::sqrt(w[c] < 1e-12 ? 0.0 : w[c]);
+    if (s == 0.0) continue;
+    for (int l = 0; l < M; ++l) {
+      const double cf = W[(size_t)c * M + l] * s;
+      if (cf == 0.0) continue;
+      const cd* src = &phi[(size_t)l * nr];
+      cd* dst = &out[(size_t)c * nr];
+      for (size_t i = 0; i < nr; ++i) dst[i] += src[i] * cf;
+    }
+  }
+}
+
 // ------------------------------------------------------- SCF state
 struct State {
   bool have = false;
@@ -1090,6 +1289,8 @@ struct State {
   vecd f[2], gf[2], U[2], W[2], UCoul, UCDir;
   vecd Uconstr;
   BCSRes bcs[2];
+  HFBRes hfb[2];
+  bool active[2] = {true, true};  // pairing of a species can collapse (HFB)
   double massCorr = MNUC;
   double energyDiff = 1.0;
 };
@@ -1127,8 +1328,14 @@ struct Scf {
   nhf_config cfg;
   State S;
   std::vector<Con> cons;
+  // HFB has replaced the BCS seed steps.  Before that the reference's E_pair
+  // reads an HFB energy that no step has set yet (iteration_data.cpp:31-33);
+  // the seed steps report the BCS pairing energy instead.
+  bool hfb_on = false;
   Scf(const nhf_config& c) : g(c.n, 0.5 * c.box_size), cfg(c) {
     S.massCorr = c.com_corr ? MNUC * (double)c.A / (double)(c.A - 1) : MNUC;
+    S.active[0] = c.pair_n.active != 0;
+    S.active[1] = c.pair_p.active != 0;
     for (int i = 0; i < c.n_con; ++i) {
       Con k;
       k.c = c.con[i];
@@ -1158,7 +1365,7 @@ struct Scf {
       return;
     }
     for (int q = 0; q < 2; ++q) {
-      if (!pairing(q).active) {
+      if (!S.active[q]) {
         S.bcs[q] = nul[q];
         continue;
       }
@@ -1184,9 +1391,49 @@ struct Scf {
     size_t N3 = g.N3;
     const double* C = cfg.edf;
     double mu = cfg.density_mix;
-    pairing_bcs(psi, eps, M);
     Dens D[2];
-    for (int q = 0; q < 2; ++q) densities(g, psi[q], M[q], S.bcs[q].v2.data(), D[q]);
+    if (cfg.pairing_type == NHF_PAIR_HFB && iter >= cfg.start_hfb_iter) {
+      hfb_on = true;
+      // updateQuantities (iteration_data.cpp:871-921): HFB from startHFBIter,
+      // its pairing tensor seeded from the last BCS solution
+      if (iter == cfg.start_hfb_iter)
+        for (int q = 0; q < 2; ++q) {
+          const int m = M[q];
+          S.hfb[q] = HFBRes();
+          S.hfb[q].kappa.assign((size_t)m * m, 0.0);
+          S.hfb[q].lambda = S.bcs[q].lambda;
+          for (int i = 0; i + 1 < m; i += 2) {
+            const double uv = std::sqrt(S.bcs[q].u2[i] * S.bcs[q].v2[i]);
+            S.hfb[q].kappa[(size_t)(i + 1) * m + i] = uv;
+            S.hfb[q].kappa[(size_t)i * m + i + 1] = -uv;
+          }
+        }
+      for (int q = 0; q < 2; ++q) {
+        if (S.active[q]) {
+          HFBRes r = hfb_species(g, psi[q], eps[q], M[q], S.hfb[q], Nq(q), pairing(q), cfg.pairing_threshold);
+          if (std::isnan(r.energy)) {
+            // keep the previous result
+          } else if (std::abs(r.energy) < 1e-12) {
+            S.active[q] = false;
+          } else {
+            S.hfb[q] = r;
+          }
+        }
+        if (S.active[q]) {
+          vecc can;
+          hfb_canonical(N3, psi[q], M[q], S.hfb[q], can);
+          vecd one(M[q], 1.0);
+          densities(g, can, M[q], one.data(), D[q]);
+        } else {
+          vecd occ(M[q], 0.0);
+          for (int i = 0; i < Nq(q) && i < M[q]; ++i) occ[i] = 1.0;
+          densities(g, psi[q], M[q], occ.data(), D[q]);
+        }
+      }
+    } else {
+      pairing_bcs(psi, eps, M);
+      for (int q = 0; q < 2; ++q) densities(g, psi[q], M[q], S.bcs[q].v2.data(), D[q]);
+    }
     auto mix = [&](vecd& old, const vecd& nw) {
       if (!S.have)
         old = nw;
@@ -1375,7 +1622,12 @@ struct Scf {
     T[NHF_E_KIN] = integ([&](size_t p) {
       return 0.5 * HBAR * HBAR * S.tau[0][p] / S.massCorr + 0.5 * HBAR * HBAR * S.tau[1][p] / S.massCorr;
     });
-    auto epair = [&](int q) { return (cfg.pairing_type == NHF_PAIR_BCS && pairing(q).active) ? S.bcs[q].Epair : 0.0; };
+    auto epair = [&](int q) {
+      if (!S.active[q]) return 0.0;
+      if (cfg.pairing_type == NHF_PAIR_BCS) return S.bcs[q].Epair;
+      if (cfg.pairing_type == NHF_PAIR_HFB) return hfb_on ? S.hfb[q].energy : S.bcs[q].Epair;
+      return 0.0;
+    };
     T[NHF_E_PAIR_N] = epair(0);
     T[NHF_E_PAIR_P] = epair(1);
     T[NHF_E_REAR] = integ([&](size_t p) {
@@ -1423,8 +1675,8 @@ struct Scf {
     double A = cfg.A, R0 = 1.2 * std::pow(A, 1.0 / 3.0);
     R->beta2 = 4 * PI * a20 / (3 * A * R0 * R0);
     R->gamma = std::atan2(std::sqrt(2.0) * a22, a20);
-    R->lambda_n = S.bcs[0].lambda;
-    R->lambda_p = S.bcs[1].lambda;
+    R->lambda_n = hfb_on ? S.hfb[0].lambda : S.bcs[0].lambda;
+    R->lambda_p = hfb_on ? S.hfb[1].lambda : S.bcs[1].lambda;
   }
 };
 
@@ -1440,6 +1692,63 @@ static void normalize(const Grid& g, vecc& psi, int m) {
   }
 }
 
+
+// ------------------------------------------------ Woods-Saxon start-up
+// Radius (radius.cpp:8-18), Y20 / Y20Grad / Y(3,0) (spherical_harmonics.cpp:
+// 63-96,155-188), DeformedWoodsSaxonPotential::getValue
+// (deformed_woods_saxon.cpp:15-20), DeformedSpinOrbitPotential::getFactor /
+// getValue (deformed_spin_orbit.cpp:12-35), restated point by point.
+struct WsPar {
+  double V0, r0, diff, kappa, V0so;
+};
+static double ws_y20_theta(double th) { return std::sqrt(5.0 / (16.0 * PI)) * (3.0 * std::cos(th) * std::cos(th) - 1.0); }
+static double ws_radius(double R0, double beta, double x, double y, double z) {
+  double th = std::acos(z / std::sqrt(x * x + y * y + z * z));
+  return R0 * (1.0 + beta * ws_y20_theta(th));
+}
+// Y20Grad as the reference writes it: the vector starts as setIdentity()
+// of a 3-vector, i.e. (1, 0, 0), so the y and z products vanish.
+static void ws_y20grad(double x, double y, double z, double g[3]) {
+  double r2 = x * x + y * y + z * z, r = std::sqrt(r2), th = std::acos(z / r);
+  double v[3] = {1.0, 0.0, 0.0};
+  for (double& e : v) e = e * 1.0 / (r2 * std::sin(th));
+  for (double& e : v) e = e * std::sqrt(5.0 / (16.0 * PI)) * (-6.0 * std::sin(th) * std::cos(th));
+  g[0] = -v[0] * x * z;
+  g[1] = -v[1] * y * z;
+  g[2] = v[2] * (x * x + y * y);
+}
+static double ws_y30(double x, double y, double z) {
+  double r = std::sqrt(x * x + y * y + z * z), c = std::cos(std::acos(z / r));
+  // associatedLegendre(3, 0, c) by the upward recurrence
+  double p0 = 1.0, p1 = c, p2 = (3.0 * c * p1 - 1.0 * p0) / 2.0, p3 = (5.0 * c * p2 - 2.0 * p1) / 3.0;
+  return std::sqrt(7.0 / (4.0 * PI)) * p3;
+}
+static void ws_fields(const Grid& g, int A, int Z, double beta2, double beta3, bool so, const WsPar& P, Fields& F) {
+  const size_t N3 = g.N3;
+  F.f.assign(N3, -1.0 / (-2.0 * MNUC / (HBAR * HBAR)));
+  F.gf.assign(3 * N3, 0.0);
+  F.U.assign(N3, 0.0);
+  F.W.assign(3 * N3, 0.0);
+  const double R0 = P.r0 * std::pow(A, 1.0 / 3.0);
+  const double ls = -35.0 * std::pow(HBAR * 0.5 / MNUC, 2);
+  for (int k = 0; k < g.n; ++k)
+    for (int j = 0; j < g.n; ++j)
+      for (int i = 0; i < g.n; ++i) {
+        const double x = g.x[i], y = g.x[j], z = g.x[k];
+        const size_t p = g.idx(i, j, k);
+        const double locR = ws_radius(R0, beta2, x, y, z);
+        const double r = std::sqrt(x * x + y * y + z * z);
+        const double b3 = beta3 * P.r0 * std::pow(A, 1.0 / 3.0) * ws_y30(x, y, z);
+        F.U[p] = -P.V0 * (1 + P.kappa * (A - 2 * Z) / ((double)A)) / (1 + std::exp((r - (locR + b3)) / P.diff));
+        if (!so || !(r > 1e-12)) continue;
+        const double gr[3] = {x / r, y / r, z / r};
+        double yg[3];
+        ws_y20grad(x, y, z, yg);
+        const double t = (r - locR) / P.diff, et = std::exp(t);
+        for (int c = 0; c < 3; ++c)
+          F.W[c * N3 + p] = ls * ((P.V0so / P.diff) * std::pow(1 + et, -2) * et * (gr[c] - R0 * beta2 * yg[c]));
+      }
+}
 }  // namespace orc
 
 using namespace orc;
@@ -1669,23 +1978,16 @@ int orc_ho_guess(int n, double box, int nev, double b, double* out) {
 
 // Full SCF loop (main.cpp:199-342) in FP64.  psi_n/psi_p in/out
 // (2n^3 x M), trace rows [E_int, E_HF, max|deps|, seconds].
-int orc_scf_run(const nhf_config* cfg, double* psi_n, double* psi_p, double* eps_n, double* eps_p, nhf_result* res,
-                double* trace, int trace_cap) {
-  Scf scf(*cfg);
+// One run of the HF loop (main.cpp:199-342) on a persistent Scf state
+// (mixing history, pairing and constraint state carry over between calls,
+// as IterationData and the constraint objects do across the points of a
+// deformation curve, main.cpp:151-366); the conjugate directions and the
+// energy history start empty.  psi/eps in/out.
+static int scf_loop(Scf& scf, vecc psi[2], vecd eps[2], nhf_result* res, double* trace, int trace_cap) {
+  const nhf_config* cfg = &scf.cfg;
   Grid& g = scf.g;
   size_t nr = 2 * g.N3;
   int M[2] = {cfg->M_n, cfg->M_p};
-  vecc psi[2];
-  psi[0].assign((const cd*)psi_n, (const cd*)psi_n + (size_t)M[0] * nr);
-  psi[1].assign((const cd*)psi_p, (const cd*)psi_p + (size_t)M[1] * nr);
-  vecd eps[2];
-  eps[0].assign(M[0], 0.0);
-  eps[1].assign(M[1], 0.0);
-  // the reference enters the loop with the start-up eigenvalues; for a
-  // guess without them the occupation-weighted quantities of iteration 0
-  // only need eps for qpEnergies and the first BCS lambda.
-  for (int q = 0; q < 2; ++q)
-    for (int i = 0; i < M[q]; ++i) eps[q][i] = (q == 0 ? eps_n : eps_p)[i];
   vecc conj[2];
   int pc[2] = {0, 0};
   double Eint_prev = 0.0, Ehf_prev = 0.0;
@@ -1778,6 +2080,66 @@ int orc_scf_run(const nhf_config* cfg, double* psi_n, double* psi_p, double* eps
   scf.observables(res);
   res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   res->seconds_to_stop = tstop;
+  return 0;
+}
+
+// Full SCF loop (main.cpp:199-342) in FP64.  psi_n/psi_p in/out
+// (2n^3 x M), trace rows [E_int, E_HF, max|deps|, seconds].
+int orc_scf_run(const nhf_config* cfg, double* psi_n, double* psi_p, double* eps_n, double* eps_p, nhf_result* res,
+                double* trace, int trace_cap) {
+  Scf scf(*cfg);
+  size_t nr = 2 * scf.g.N3;
+  int M[2] = {cfg->M_n, cfg->M_p};
+  vecc psi[2];
+  vecd eps[2];
+  for (int q = 0; q < 2; ++q) {
+    const double* ps = q ? psi_p : psi_n;
+    const double* ep = q ? eps_p : eps_n;
+    psi[q].assign((const cd*)ps, (const cd*)ps + (size_t)M[q] * nr);
+    eps[q].assign(ep, ep + M[q]);
+  }
+  int rc = scf_loop(scf, psi, eps, res, trace, trace_cap);
+  if (rc != 0) return rc;
+  std::copy(psi[0].begin(), psi[0].end(), (cd*)psi_n);
+  std::copy(psi[1].begin(), psi[1].end(), (cd*)psi_p);
+  std::copy(eps[0].begin(), eps[0].end(), eps_n);
+  std::copy(eps[1].begin(), eps[1].end(), eps_p);
+  return 0;
+}
+
+// Expectation value of a constraint operator (the Q of getField, e.g.
+// quadrupole_constraint.cpp:21-46): trapezoid integral of O_kind * rho.
+double orc_moment(int n, double a, int kind, const double* rho) {
+  Grid g(n, a);
+  vecd O, t(g.N3);
+  op_field(g, kind, O);
+  for (size_t p = 0; p < g.N3; ++p) t[p] = O[p] * rho[p];
+  return integral(g, t);
+}
+
+// Deformation curve (main.cpp:122-191 and the loop to :366): the points run
+// one after the other on one Scf state, each started from the previous
+// point's states; the last constraint (NHF_CON_Q20) takes target mu20[i]
+// (main.cpp:186-189).  res: npts results.
+int orc_curve_run(const nhf_config* cfg, const double* mu20, int npts, double* psi_n, double* psi_p, double* eps_n,
+                  double* eps_p, nhf_result* res) {
+  if (cfg->n_con < 1 || cfg->con[cfg->n_con - 1].kind != NHF_CON_Q20 || npts < 1) return -1;
+  Scf scf(*cfg);
+  size_t nr = 2 * scf.g.N3;
+  int M[2] = {cfg->M_n, cfg->M_p};
+  vecc psi[2];
+  vecd eps[2];
+  for (int q = 0; q < 2; ++q) {
+    const double* ps = q ? psi_p : psi_n;
+    const double* ep = q ? eps_p : eps_n;
+    psi[q].assign((const cd*)ps, (const cd*)ps + (size_t)M[q] * nr);
+    eps[q].assign(ep, ep + M[q]);
+  }
+  for (int i = 0; i < npts; ++i) {
+    scf.cons.back().c.target = mu20[i];
+    int rc = scf_loop(scf, psi, eps, res + i, nullptr, 0);
+    if (rc != 0) return rc;
+  }
   std::copy(psi[0].begin(), psi[0].end(), (cd*)psi_n);
   std::copy(psi[1].begin(), psi[1].end(), (cd*)psi_p);
   std::copy(eps[0].begin(), eps[0].end(), eps_n);
@@ -1968,4 +2330,62 @@ int orc_lcp_loss_grad(int n, int batch, const float* in8, const float* tgt4, con
 int orc_sizeof_config() { return (int)sizeof(nhf_config); }
 int orc_sizeof_result() { return (int)sizeof(nhf_result); }
 
+// HFB pieces for the checks: kappa(r) and Delta of the window [start, end]
+// of an already time-reversal-completed basis, and the lambda bisection
+int orc_hfb_fields(int n, double a, const double* ph, int M, const double* f, const double* kappa, int start,
+                   int end, double V0, double* kr, double* D) {
+  Grid g(n, a);
+  vecc P((const cd*)ph, (const cd*)ph + (size_t)M * 2 * g.N3);
+  vecd fv(f, f + M), kv(kappa, kappa + (size_t)M * M), k, d;
+  hfb_fields(g, P, M, fv, kv, start, end, V0, k, d);
+  std::copy(k.begin(), k.end(), kr);
+  std::copy(d.begin(), d.end(), D);
+  return 0;
+}
+double orc_hfb_solve(int nS, const double* e, const double* D, double lambda0, double target, int start, double* U,
+                     double* V, double* lambda) {
+  vecd d(D, D + (size_t)nS * nS), u, v;
+  double l = 0.0;
+  const double ep = hfb_solve(nS, e, d, lambda0, target, start, u, v, l);
+  std::copy(u.begin(), u.end(), U);
+  std::copy(v.begin(), v.end(), V);
+  *lambda = l;
+  return ep;
+}
+
+
+void orc_ws_fields(int n, double a, int A, int Z, double beta2, double beta3, int so, const double* par, double* f,
+                   double* gf, double* U, double* W) {
+  Grid g(n, a);
+  Fields F;
+  ws_fields(g, A, Z, beta2, beta3, so != 0, WsPar{par[0], par[1], par[2], par[3], par[4]}, F);
+  std::copy(F.f.begin(), F.f.end(), f);
+  std::copy(F.gf.begin(), F.gf.end(), gf);
+  std::copy(F.U.begin(), F.U.end(), U);
+  std::copy(F.W.begin(), F.W.end(), W);
+}
+
+// main.cpp:72-116: oscillator guess (length 1.2 A^0.3333333) of
+// max(M_n, M_p) columns, one GCG call on the Woods-Saxon Hamiltonian, the
+// first M_q columns per species, trapezoid-normalised.
+int orc_ws_guess(const nhf_config* cfg, double beta2, double beta3, const double* par, const nhf_gcg_params* prm,
+                 double* psi_n, double* psi_p, double* eps_n, double* eps_p) {
+  Grid g(cfg->n, 0.5 * cfg->box_size);
+  Fields F;
+  ws_fields(g, cfg->A, cfg->Z, beta2, beta3, cfg->spin_orbit != 0, WsPar{par[0], par[1], par[2], par[3], par[4]}, F);
+  const int m = std::max(cfg->M_n, cfg->M_p);
+  const size_t nr = 2 * g.N3;
+  vecc X((size_t)m * nr);
+  orc_ho_guess(cfg->n, cfg->box_size, m, std::pow(cfg->A, 0.3333333) * 1.20, (double*)X.data());
+  vecc Xo;
+  vecd lo;
+  if (!gcg(g, F, X, m, vecc(), 0, *prm, Xo, lo)) return -1;
+  normalize(g, Xo, m);
+  for (int q = 0; q < 2; ++q) {
+    const int M = q ? cfg->M_p : cfg->M_n;
+    std::copy(Xo.begin(), Xo.begin() + (size_t)M * nr, (cd*)(q ? psi_p : psi_n));
+    std::copy(lo.begin(), lo.begin() + M, q ? eps_p : eps_n);
+  }
+  return 0;
+}
 }  // extern "C"

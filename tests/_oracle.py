@@ -28,6 +28,7 @@ def lib():
         _lib = ctypes.CDLL(LIB)
         _lib.orc_integral.restype = ctypes.c_double
         _lib.orc_build_csr.restype = ctypes.c_long
+        _lib.orc_moment.restype = ctypes.c_double
     return _lib
 
 
@@ -176,6 +177,31 @@ def ho_guess(n, box, nev, b):
     return out
 
 
+WS_FILE = (55.5, 1.20, 0.67, 0.86, 55.5)
+
+
+def ws_fields(n, box, A, Z, beta2=0.0, beta3=0.0, so=True, par=WS_FILE):
+    N3 = n ** 3
+    f, U, gf, W = np.zeros(N3), np.zeros(N3), np.zeros(3 * N3), np.zeros(3 * N3)
+    lib().orc_ws_fields(n, ctypes.c_double(0.5 * box), A, Z, ctypes.c_double(beta2), ctypes.c_double(beta3), int(so),
+                        dp(np.array(par, dtype=np.float64)), dp(f), dp(gf), dp(U), dp(W))
+    return f, gf, U, W
+
+
+def ws_guess(cfg: NhfConfig, beta2=0.0, beta3=0.0, par=WS_FILE, params=None):
+    if params is None:
+        params = NhfGcgParams(10, 40, 1e-2, 1e-4)
+    rows = 2 * cfg.n ** 3
+    psi_n = np.zeros((rows, cfg.M_n), dtype=np.complex128, order="F")
+    psi_p = np.zeros((rows, cfg.M_p), dtype=np.complex128, order="F")
+    eps_n, eps_p = np.zeros(cfg.M_n), np.zeros(cfg.M_p)
+    rc = lib().orc_ws_guess(ctypes.byref(cfg), ctypes.c_double(beta2), ctypes.c_double(beta3),
+                            dp(np.array(par, dtype=np.float64)), ctypes.byref(params), dp(psi_n), dp(psi_p),
+                            dp(eps_n), dp(eps_p))
+    assert rc == 0, rc
+    return psi_n, psi_p, eps_n, eps_p
+
+
 def scf_run(cfg: NhfConfig, psi_n, psi_p, eps_n, eps_p, trace_cap=2000):
     psi_n = cplx_cols(psi_n).copy(order="F")
     psi_p = cplx_cols(psi_p).copy(order="F")
@@ -187,6 +213,22 @@ def scf_run(cfg: NhfConfig, psi_n, psi_p, eps_n, eps_p, trace_cap=2000):
                            dp(trace), trace_cap)
     assert rc == 0, rc
     return psi_n, psi_p, eps_n, eps_p, res, trace[: res.iterations]
+
+
+def moment(n, a, kind, rho):
+    return lib().orc_moment(n, ctypes.c_double(a), kind, dp(np.ascontiguousarray(rho, dtype=np.float64)))
+
+
+def curve_run(cfg: NhfConfig, mu20, psi_n, psi_p, eps_n, eps_p):
+    psi_n = cplx_cols(psi_n).copy(order="F")
+    psi_p = cplx_cols(psi_p).copy(order="F")
+    eps_n = np.array(eps_n, dtype=np.float64)
+    eps_p = np.array(eps_p, dtype=np.float64)
+    mu = np.ascontiguousarray(mu20, dtype=np.float64)
+    res = (NhfResult * len(mu))()
+    rc = lib().orc_curve_run(ctypes.byref(cfg), dp(mu), len(mu), dp(psi_n), dp(psi_p), dp(eps_n), dp(eps_p), res)
+    assert rc == 0, rc
+    return (psi_n, psi_p, eps_n, eps_p), list(res)
 
 
 def fields_once(cfg: NhfConfig, psi_n, psi_p, eps_n, eps_p):
@@ -239,3 +281,31 @@ def lcp_random_case(n=5, batch=2, seed=3):
         parts.append(rng.standard_normal(27 * ci[l] * co[l]) * np.sqrt(2.0 / (27 * ci[l])))
         parts.append(rng.standard_normal(co[l]) * 0.1)
     return x8, t4, np.concatenate(parts)
+
+
+def hfb_fields(n, a, ph, f, kappa, start, end, V0):
+    """kappa(r) and Delta (window) of the two-basis HFB from a time-reversal
+    completed basis ph (complex (2 n^3, M)), Fermi factors f, kappa (M x M)."""
+    ph = cplx_cols(ph)
+    M = ph.shape[1]
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    kappa = np.asfortranarray(kappa, dtype=np.float64)
+    nS = end - start + 1
+    kr = np.zeros(n ** 3)
+    D = np.zeros((nS, nS), order="F")
+    lib().orc_hfb_fields(n, ctypes.c_double(a), dp(ph), M, dp(f), dp(kappa), start, end, ctypes.c_double(V0), dp(kr),
+                         dp(D))
+    return kr, D
+
+
+def hfb_solve(e, D, lambda0, target, start=0):
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    D = np.asfortranarray(D, dtype=np.float64)
+    nS = e.size
+    U = np.zeros((nS, nS), order="F")
+    V = np.zeros((nS, nS), order="F")
+    lam = ctypes.c_double()
+    lib().orc_hfb_solve.restype = ctypes.c_double
+    ep = lib().orc_hfb_solve(nS, dp(e), dp(D), ctypes.c_double(lambda0), ctypes.c_double(target), start, dp(U),
+                             dp(V), ctypes.byref(lam))
+    return ep, U, V, lam.value

@@ -1,0 +1,112 @@
+"""Multipole constraints and the deformation curve in the oracle (reference
+src/skyrme/*_constraint.cpp, main.cpp:122-191): the constraint operators
+against closed-form moments of a displaced anisotropic Gaussian, the
+augmented-Lagrangian penalty reaching its target in a self-consistent run
+(checked through moments computed here from the final states), and the
+warm-started curve reaching the same constrained fixed point as a cold run.
+CPU only; the device parity tests are in test_gpu_constraints.py."""
+import math
+
+import numpy as np
+import pytest
+
+import _cases as K
+import _oracle as O
+from nuclearcgm_amd import config as C
+
+
+def test_constraint_operators_closed_form():
+    """rho = N g(x; 0, sx) g(y; y0, sy) g(z; z0, sz) well inside the box:
+    <x> = 0, <y> = N y0, <z> = N z0, <x^2 - y^2> = N (sx^2 - sy^2 - y0^2),
+    <2xy> = 0, <Q20> = sqrt(5/16pi) N (2 (sz^2 + z0^2) - sx^2 - sy^2 - y0^2),
+    <r^3 Y30> = sqrt(7/16pi) N (2 E[z^3] - 3 z0 (sx^2 + sy^2 + y0^2))."""
+    n, box = 50, 30.0
+    a, h, X, Y, Z = K.grid(n, box)
+    sx, sy, sz, y0, z0, N = 1.2, 1.4, 1.0, 0.4, -0.7, 12.0
+
+    def g(u, c, s):
+        return np.exp(-0.5 * ((u - c) / s) ** 2) / (s * math.sqrt(2 * math.pi))
+
+    rho = N * g(X, 0.0, sx) * g(Y, y0, sy) * g(Z, z0, sz)
+    ez3 = z0 ** 3 + 3 * z0 * sz ** 2
+    want = {
+        C.CON_XCM: 0.0,
+        C.CON_YCM: N * y0,
+        C.CON_ZCM: N * z0,
+        C.CON_X2MY2: N * (sx ** 2 - sy ** 2 - y0 ** 2),
+        C.CON_XY2: 0.0,
+        C.CON_Q20: math.sqrt(5 / (16 * math.pi)) * N * (2 * (sz ** 2 + z0 ** 2) - sx ** 2 - sy ** 2 - y0 ** 2),
+        C.CON_OCT: math.sqrt(7 / (16 * math.pi)) * N * (2 * ez3 - 3 * z0 * (sx ** 2 + sy ** 2 + y0 ** 2)),
+    }
+    for kind, v in want.items():
+        got = O.moment(n, a, kind, rho)
+        assert abs(got - v) < 1e-9 * N * 10, (kind, got, v)
+
+
+def _o16(max_iter=400):
+    return K.o16_config(n=16, box=16.0, M=8, max_iter=max_iter)
+
+
+def _moments(cfg, psi_n, psi_p):
+    """Moments of the total density of the final occupied states (no
+    pairing: the lowest N and Z states)."""
+    a = 0.5 * cfg.box_size
+    N = cfg.A - cfg.Z
+    rho = O.densities(cfg.n, a, psi_n[:, :N], np.ones(N))[0] + O.densities(cfg.n, a, psi_p[:, :cfg.Z],
+                                                                             np.ones(cfg.Z))[0]
+    _, _, X, Y, Z = K.grid(cfg.n, cfg.box_size)
+    w1 = np.ones(cfg.n)
+    w1[0] = w1[-1] = 0.5
+    wt = (w1[None, None, :] * w1[None, :, None] * w1[:, None, None]).reshape(-1) * (2 * a / (cfg.n - 1)) ** 3
+    q20 = math.sqrt(5 / (16 * math.pi)) * (2 * Z ** 2 - X ** 2 - Y ** 2)
+    return {"N": wt @ rho, "x": wt @ (X * rho), "y": wt @ (Y * rho), "z": wt @ (Z * rho),
+            "x2-y2": wt @ ((X ** 2 - Y ** 2) * rho), "q20": wt @ (q20 * rho)}
+
+
+@pytest.fixture(scope="module")
+def constrained_runs():
+    """O-16 on 16^3: unconstrained, and with the curve's constraint list at
+    beta = 0.1 (stiffness 0.005 as in the reference)."""
+    cfg0 = _o16()
+    g = K.guess(cfg0, ho=O.ho_guess)
+    free = O.scf_run(cfg0, *g)
+    cfg = C.set_curve_constraints(_o16(), C.mu20_from_beta(0.1, 16))
+    con = O.scf_run(cfg, *g)
+    return cfg, g, free, con
+
+
+def test_quadrupole_penalty_reaches_target(constrained_runs):
+    cfg, _, free, con = constrained_runs
+    mu = C.mu20_from_beta(0.1, 16)
+    r = con[4]
+    assert r.converged and free[4].converged
+    m = _moments(cfg, con[0], con[1])
+    print("moments", m, "beta2", r.beta2, "E", r.terms[C.E_INT], "free E", free[4].terms[C.E_INT])
+    # reference stop rule: relative error of Q20 below constraintsTol 1e-3
+    assert abs(m["q20"] / mu - 1) < 2e-3
+    assert abs(r.beta2 - 0.1) < 2e-4
+    # centre of mass at the origin, no triaxiality
+    assert max(abs(m["x"]), abs(m["y"]), abs(m["z"])) < 1e-4 * m["N"]
+    assert abs(m["x2-y2"]) < 1e-2
+    # O-16 is spherical: the deformed configuration costs energy
+    assert r.terms[C.E_INT] > free[4].terms[C.E_INT] + 0.1
+    assert abs(free[4].beta2) < 1e-4
+
+
+def test_curve_warm_start_reaches_cold_fixed_point(constrained_runs):
+    """Three curve points beta = 0, 0.05, 0.1 warm-started from each other
+    (mixing, pairing and multiplier state carried over): every point meets
+    its target and the last one sits at the fixed point of the cold
+    constrained run at beta = 0.1."""
+    cfg, g, _, con = constrained_runs
+    mus = C.curve_targets(0.0, 0.1, 0.05, 16)
+    assert len(mus) == 3 and abs(mus[-1] - C.mu20_from_beta(0.1, 16)) < 1e-12
+    last, res = O.curve_run(cfg, mus, *g)
+    for beta, r in zip((0.0, 0.05, 0.1), res):
+        print("curve point", beta, r.iterations, r.converged, r.terms[C.E_INT], r.beta2)
+        assert r.converged
+        assert abs(r.beta2 - beta) < 2e-4
+    assert abs(res[-1].terms[C.E_INT] - con[4].terms[C.E_INT]) < 1e-4
+    assert np.abs(last[2] - con[2]).max() < 1e-3
+    # later points start from a neighbouring solution: fewer iterations
+    assert res[2].iterations < con[4].iterations

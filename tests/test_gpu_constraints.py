@@ -1,0 +1,90 @@
+"""Constraints, the deformation curve and concurrent (batched) problems on
+the device against the oracle: O-16 on 16^3 with the curve's constraint
+list (COM x/y/z, x^2-y^2, 2xy, octupole with target 0, then Q20).
+
+Constrained runs stop when |dE| < 1e-6 MeV and every constraint's relative
+error is below 1e-3; at that point the oracle's own cold and warm-started
+runs at beta = 0.1 agree to 5e-6 MeV.  The device runs (whose FP64 sums use
+atomics, so they are not bitwise reproducible) are held to 5e-5 MeV."""
+import time
+
+import numpy as np
+import pytest
+
+import _cases as K
+import _oracle as O
+from nuclearcgm_amd import config as C
+
+pytestmark = pytest.mark.gpu
+
+BETAS = (0.0, 0.05, 0.1)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from nuclearcgm_amd import hip
+
+    c = hip.Context(0)
+    yield c
+    c.close()
+
+
+def _cfg(beta):
+    cfg = K.o16_config(n=16, box=16.0, M=8, max_iter=400)
+    return C.set_curve_constraints(cfg, C.mu20_from_beta(beta, 16))
+
+
+@pytest.fixture(scope="module")
+def guess():
+    return K.guess(_cfg(0.0), ho=O.ho_guess)
+
+
+def test_constrained_run_matches_oracle(ctx, guess):
+    cfg = _cfg(0.1)
+    ref = O.scf_run(cfg, *guess)
+    got = ctx.problem(cfg).scf_run(*guess)
+    ro, rg = ref[4], got[4]
+    print("oracle", ro.iterations, ro.terms[C.E_INT], ro.beta2, "gpu", rg.iterations, rg.terms[C.E_INT], rg.beta2)
+    assert rg.converged and ro.converged
+    assert abs(rg.terms[C.E_INT] - ro.terms[C.E_INT]) < 5e-5
+    assert abs(rg.terms[C.E_HF] - ro.terms[C.E_HF]) < 5e-5
+    assert abs(rg.beta2 - ro.beta2) < 2e-5 and abs(rg.beta2 - 0.1) < 2e-4
+    assert np.abs(got[2] - ref[2]).max() < 1e-3
+
+
+def test_curve_matches_oracle(ctx, guess):
+    cfg = _cfg(0.0)
+    mus = C.curve_targets(0.0, 0.1, 0.05, 16)
+    _, ref = O.curve_run(cfg, mus, *guess)
+    (pn, pp, en, ep), got = ctx.problem(cfg).curve_run(mus, *guess)
+    for beta, ro, rg in zip(BETAS, ref, got):
+        print(f"beta {beta}: oracle {ro.iterations} {ro.terms[C.E_INT]:.7f}  gpu {rg.iterations} "
+              f"{rg.terms[C.E_INT]:.7f} beta2 {rg.beta2:.6f}")
+        assert rg.converged
+        assert abs(rg.terms[C.E_INT] - ro.terms[C.E_INT]) < 5e-5
+        assert abs(rg.beta2 - beta) < 2e-4
+    # energies rise with the imposed deformation of the spherical O-16
+    e = [r.terms[C.E_INT] for r in got]
+    assert e[0] < e[1] < e[2]
+
+
+def test_batch_matches_solo_runs(ctx, guess):
+    """Three constrained problems run concurrently give the results of the
+    same problems run one at a time."""
+    solo, t_solo = [], 0.0
+    for b in BETAS:
+        p = ctx.problem(_cfg(b))
+        t = time.perf_counter()
+        solo.append(p.scf_run(*guess))
+        t_solo += time.perf_counter() - t
+    probs = [ctx.problem(_cfg(b)) for b in BETAS]
+    t = time.perf_counter()
+    batch = ctx.batch_run([(p, guess) for p in probs])
+    t_batch = time.perf_counter() - t
+    print(f"one at a time {t_solo:.3f} s, batched {t_batch:.3f} s")
+    for b, s, r in zip(BETAS, solo, batch):
+        print(b, s[4].iterations, s[4].terms[C.E_INT], r[4].iterations, r[4].terms[C.E_INT])
+        assert r[4].converged
+        assert abs(r[4].terms[C.E_INT] - s[4].terms[C.E_INT]) < 5e-5
+        assert abs(r[4].beta2 - b) < 2e-4
+        assert len(r[5]) == r[4].iterations

@@ -103,12 +103,13 @@ def test_scf_o16_matches_oracle(H):
     assert abs(r_or[4].r2_p - r_gp[4].r2_p) < 1e-5
 
 
-@pytest.mark.parametrize("npts,ka,kb", [(1000, 5, 7), (27000, 40, 33), (64000, 96, 96)])
+@pytest.mark.parametrize("npts,ka,kb", [(1000, 5, 7), (27000, 40, 33), (64000, 96, 96), (20001, 150, 130)])
 def test_block_gram(H, npts, ka, kb):
     """The GCG block product A^H B: FP64 kernel to rounding; split-BF16 MFMA
-    kernel on FP32 data to 3e-8 of |A||B| (three-way split, FP64 sums every
-    16 points), both for a general product and the Hermitian lower-block
-    variant."""
+    kernel on FP32 data to 3e-8 of |A||B| (exact three-way split, FP64 sums
+    every 16 points), both for a general product and the Hermitian
+    lower-block variant; 150 x 130 spans several 96 x 96 super-tiles and an
+    odd point count leaves a partial slab."""
     _, ctx = H
     rng = np.random.default_rng(npts)
     A = rng.standard_normal((2 * npts, ka)) + 1j * rng.standard_normal((2 * npts, ka))

@@ -383,3 +383,78 @@ def test_lcp_oracle_forward_layout():
     x8[0, :, :4] *= -1
     y = O.lcp_forward(n, x8, w)[0].reshape(n, n, n, 4)
     assert np.abs(y + 1e-3 * exp).max() < 1e-9
+
+
+# --------------------------------------------------------------------- HFB
+def test_hfb_fields_single_kramers_pair():
+    """phi_0 = (g, 0), phi_1 = T phi_0 = (0, -g) with g real:
+    kappa(r) = 2 f0 f1 kappa_01 g^2 and
+    Delta_01 = -V0 f0^2 f1^2 kappa_01 int g^4 (trapezoid), Delta_10 = -Delta_01."""
+    n, a = 12, 6.0
+    x = np.linspace(-a, a, n)
+    X = x[None, None, :]; Y = x[None, :, None]; Z = x[:, None, None]
+    g = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 4.0).reshape(-1)
+    ph = np.zeros((2 * n ** 3, 2), dtype=complex)
+    ph[0::2, 0] = g
+    ph[1::2, 1] = -g
+    f = np.array([0.9, 0.7])
+    k01 = 0.3
+    kappa = np.array([[0.0, k01], [-k01, 0.0]])
+    V0 = 250.0
+    kr, D = O.hfb_fields(n, a, ph, f, kappa, 0, 1, V0)
+    assert np.abs(kr - 2 * f[0] * f[1] * k01 * g ** 2).max() < 1e-13
+    h = x[1] - x[0]
+    w1 = np.ones(n); w1[0] = w1[-1] = 0.5
+    w = (w1[None, None, :] * w1[None, :, None] * w1[:, None, None]).reshape(-1) * h ** 3
+    d01 = -V0 * f[0] ** 2 * f[1] ** 2 * k01 * np.dot(w, g ** 4)
+    assert abs(D[0, 1] - d01) < 1e-10 * abs(d01)
+    assert D[1, 0] == -D[0, 1] and D[0, 0] == 0 and D[1, 1] == 0
+
+
+def test_hfb_solve_pair_diagonal_is_bcs():
+    """Delta coupling only Kramers partners (2i, 2i+1): the two-basis HFB
+    eigenproblem decouples into BCS pairs, v^2 = (1 - xi/E)/2,
+    E = sqrt(xi^2 + d^2); the bisection must meet sum 2 v^2 = N and
+    |E_pair| = sum d^2 / (2 E)."""
+    e_pairs = np.array([-30.0, -22.0, -15.0, -11.0, -9.5, -7.0, -3.0, 1.0])
+    d = np.array([0.5, 0.8, 1.1, 1.4, 1.3, 1.2, 0.9, 0.6])
+    nS = 2 * e_pairs.size
+    e = np.repeat(e_pairs, 2)
+    D = np.zeros((nS, nS))
+    for i, di in enumerate(d):
+        D[2 * i, 2 * i + 1] = di
+        D[2 * i + 1, 2 * i] = -di
+    N = 10
+    ep, U, V, lam = O.hfb_solve(e, D, -9.0, N)
+    xi = e_pairs - lam
+    E = np.sqrt(xi ** 2 + d ** 2)
+    v2 = 0.5 * (1 - xi / E)
+    assert abs(2 * v2.sum() - N) < 1e-9
+    assert abs((V ** 2).sum() - N) < 1e-11
+    assert abs(abs(ep) - (d ** 2 / (2 * E)).sum()) < 1e-9
+    # the canonical occupations: eigenvalues of V V^T come in pairs v2_i
+    occ = np.sort(np.linalg.eigvalsh(V @ V.T))
+    assert np.abs(occ - np.sort(np.repeat(v2, 2))).max() < 1e-9
+
+
+def test_hfb_zero_pairing_reduces_to_hf():
+    """A vanishing pairing strength: the BCS seed steps keep the HF
+    occupations, the HFB pairing field collapses (|Delta| < 1e-3) and the
+    species falls back to the first N states: the HF trajectory."""
+    cfg = K.o16_config(n=12, box=14.0, M=10, max_iter=8)
+    cfg.max_iter = 8
+    g = K.guess(cfg, ho=O.ho_guess)
+    ref = O.scf_run(cfg, *g)
+    cfg.pairing_type = C.PAIR_HFB
+    cfg.start_hfb_iter = 2
+    cfg.pair_n = C.NhfPairing(V0=1e-8, window=5.0, window_both=1, active=1)
+    cfg.pair_p = C.NhfPairing(V0=1e-8, window=5.0, window_both=1, active=1)
+    hfb = O.scf_run(cfg, *g)
+    # pairing runs reorder the states into Kramers pairs after each GCG
+    # step, which changes the rounding of the following steps: 1e-6 MeV
+    # against HF, while a BCS run of the same strength (same ordering path,
+    # the same HF occupations) agrees to rounding
+    assert np.abs(hfb[5][:, 0] - ref[5][:, 0]).max() < 1e-6
+    cfg.pairing_type = C.PAIR_BCS
+    bcs = O.scf_run(cfg, *g)
+    assert np.abs(hfb[5][:, 0] - bcs[5][:, 0]).max() < 1e-10

@@ -1,0 +1,143 @@
+"""Woods-Saxon start-up (reference main.cpp:72-116): the oracle's field
+restatement pinned by closed forms, the host library against the oracle,
+and the oracle's start-up solve pinned by a dense diagonalisation and by the
+spin-orbit level order.  CPU only; the device solve is in test_gpu_ws.py."""
+import numpy as np
+import pytest
+
+import _cases as K
+import _oracle as O
+from nuclearcgm_amd import config as C
+
+HB2M = C.HBAR ** 2 / (2 * C.MNUC)
+LS = -35.0 * (0.5 * C.HBAR / C.MNUC) ** 2
+V0, R0C, DIFF, KAPPA, V0SO = O.WS_FILE
+
+
+def _ylm0(l, ct):
+    """Y_l0 from numpy's Legendre series (independent of the oracle)."""
+    c = np.zeros(l + 1)
+    c[l] = 1.0
+    return np.sqrt((2 * l + 1) / (4 * np.pi)) * np.polynomial.legendre.legval(ct, c)
+
+
+def _radius(X, Y, Z, A, beta2, beta3=0.0):
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    R0 = R0C * A ** (1.0 / 3.0)
+    return R0 * (1 + beta2 * _ylm0(2, Z / r)) + beta3 * R0 * _ylm0(3, Z / r)
+
+
+CASES = [(40, 20, 0.0, 0.0), (48, 20, 0.3, 0.0), (40, 20, -0.2, 0.1), (24, 12, 0.25, 0.05)]
+
+
+@pytest.mark.parametrize("A,Z,b2,b3", CASES)
+def test_oracle_ws_central_surface(A, Z, b2, b3):
+    """Inverting U = -V (1 + exp((r - R)/a))^-1 point by point gives the
+    surface R(theta) = R0 (1 + b2 Y20) + b3 R0 Y30 with
+    V = V0 (1 + kappa (A - 2Z)/A); the kinetic field is hbar^2/2m, grad f = 0."""
+    n, box = 14, 16.0
+    f, gf, U, W = O.ws_fields(n, box, A, Z, b2, b3)
+    _, _, X, Y, Zc = K.grid(n, box)
+    depth = V0 * (1 + KAPPA * (A - 2 * Z) / A)
+    r = np.sqrt(X ** 2 + Y ** 2 + Zc ** 2)
+    ok = -U > 1e-8 * depth
+    R_est = r[ok] - DIFF * np.log(depth / -U[ok] - 1.0)
+    assert np.abs(R_est - _radius(X, Y, Zc, A, b2, b3)[ok]).max() < 1e-7
+    assert np.all(U < 0) and np.all(U > -depth)
+    assert np.allclose(f, HB2M, rtol=1e-15) and not gf.any()
+
+
+@pytest.mark.parametrize("b2", [0.0, 0.3, -0.15])
+def test_oracle_ws_spin_orbit_field(b2):
+    """W = ls g (r_hat - grad R): the y, z components give ls g, which must be
+    the logistic derivative of the spin-orbit-depth Woods-Saxon profile,
+    dU_so/dr = (-U_so / a)(1 + U_so / V0_so) (U_so from the oracle's central
+    field with kappa = 0, depth V0_so, b3 = 0); the x component carries the
+    reference's radius gradient, 6 sqrt(5/16pi) R0 b2 x z^2 / r^3 = -r dR/dx
+    (dR/dx by central differences of the Legendre-series radius)."""
+    n, box, A, Z = 12, 15.0, 40, 20
+    _, _, W = O.ws_fields(n, box, A, Z, b2, 0.0)[1:]
+    _, _, Uso, _ = O.ws_fields(n, box, A, Z, b2, 0.0, so=False, par=(V0SO, R0C, DIFF, 0.0, V0SO))
+    _, _, X, Y, Zc = K.grid(n, box)
+    N3 = n ** 3
+    r = np.sqrt(X ** 2 + Y ** 2 + Zc ** 2)
+    Wx, Wy, Wz = W[:N3], W[N3:2 * N3], W[2 * N3:]
+    lsg = LS * (-Uso / DIFF) * (1 + Uso / V0SO)
+    scale = np.abs(lsg).max()
+    for comp, c in ((Wy, Y), (Wz, Zc)):
+        assert np.abs(comp - lsg * c / r).max() < 1e-12 * scale
+    big = np.abs(lsg) > 1e-3 * scale
+    dRx = X / r - Wx / np.where(big, lsg, 1.0)
+    e = 1e-5
+    dR_dx = (_radius(X + e, Y, Zc, A, b2) - _radius(X - e, Y, Zc, A, b2)) / (2 * e)
+    assert np.abs(dRx[big] - (-r * dR_dx)[big]).max() < 1e-7
+    if b2 == 0.0:
+        assert np.abs(Wx - lsg * X / r).max() < 1e-12 * scale
+    # and the spin-orbit field is off without spin-orbit
+    assert not O.ws_fields(n, box, A, Z, b2, 0.0, so=False)[3].any()
+
+
+@pytest.mark.parametrize("A,Z,b2,b3", CASES)
+@pytest.mark.parametrize("so", [True, False])
+def test_host_ws_fields_match_oracle(A, Z, b2, b3, so):
+    from nuclearcgm_amd import hip
+
+    n, box = 14, 16.0
+    got = hip.ws_fields(n, box, A, Z, b2, b3, so)
+    ref = O.ws_fields(n, box, A, Z, b2, b3, so)
+    for a, b in zip(got, ref):
+        assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300)
+    # explicit parameters equal the defaults
+    got2 = hip.ws_fields(n, box, A, Z, b2, b3, so, C.ws_params())
+    assert all(np.array_equal(a, b) for a, b in zip(got, got2))
+
+
+def _ws_cfg(n, box, A, Z, M, so):
+    d = {"functional": "SLy4", "maxIterations": 1, "energyTol": 1e-6,
+         "box": {"boxSize": box, "axisMeshPoints": n}, "nucleus": {"A": A, "Z": Z}}
+    cfg = C.config_from_input(d)
+    cfg.M_n = cfg.M_p = M
+    cfg.spin_orbit = int(so)
+    return cfg
+
+
+def test_oracle_ws_guess_matches_dense_spectrum():
+    """Without spin-orbit the start-up Hamiltonian is a real symmetric
+    spatial operator times the spin identity: the oracle's start-up
+    eigenvalues are its lowest spatial eigenvalues, each twice (dense
+    diagonalisation of the 4th-order kinetic stencil + U on 12^3)."""
+    n, box = 12, 13.0
+    cfg = _ws_cfg(n, box, 16, 8, 8, so=False)
+    psi_n, psi_p, eps_n, eps_p = O.ws_guess(cfg, params=C.NhfGcgParams(30, 80, 1e-9, 1e-10))
+    _, h, X, Y, Zc = K.grid(n, box)
+    U = O.ws_fields(n, box, 16, 8, so=False)[2]
+    N3 = n ** 3
+    D2 = (np.diag(np.full(n, -30.0)) + np.diag(np.full(n - 1, 16.0), 1) + np.diag(np.full(n - 1, 16.0), -1)
+          + np.diag(np.full(n - 2, -1.0), 2) + np.diag(np.full(n - 2, -1.0), -2)) / (12 * h * h)
+    I = np.eye(n)
+    lap = np.kron(np.kron(I, I), D2) + np.kron(np.kron(I, D2), I) + np.kron(np.kron(D2, I), I)
+    Hs = -HB2M * lap + np.diag(U)
+    w = np.linalg.eigvalsh(Hs)[:4]
+    ref = np.repeat(w, 2)
+    print("start-up eps", eps_n, "dense", ref)
+    assert np.abs(eps_n - ref).max() < 1e-6
+    assert np.array_equal(eps_n, eps_p) and np.array_equal(psi_n, psi_p)
+    # trapezoid-normalised columns
+    w1 = np.ones(n)
+    w1[0] = w1[-1] = 0.5
+    wt = (w1[None, None, :] * w1[None, :, None] * w1[:, None, None]).reshape(-1) * h ** 3
+    dens = np.abs(psi_n[0::2]) ** 2 + np.abs(psi_n[1::2]) ** 2
+    assert np.abs(wt @ dens - 1.0).max() < 1e-12
+    assert N3 == dens.shape[0]
+
+
+def test_oracle_ws_guess_spin_orbit_order():
+    """With the spin-orbit term the 1p shell splits with p3/2 (4 states)
+    below p1/2 (2 states) -- the sign of the nuclear spin-orbit force; the
+    reference's default start-up parameters (10 GCG passes, tol 1e-2)."""
+    cfg = _ws_cfg(14, 15.0, 16, 8, 8, so=True)
+    _, _, eps, _ = O.ws_guess(cfg)
+    print("start-up eps with spin-orbit", eps)
+    assert eps[0] < eps[2] - 5.0
+    assert np.ptp(eps[:2]) < 0.05 and np.ptp(eps[2:6]) < 0.05 and np.ptp(eps[6:8]) < 0.05
+    assert eps[6] - eps[5] > 1.0
