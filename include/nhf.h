@@ -280,6 +280,13 @@ int nhf_batch_run(void *ctx, nhf_batch_item *items, int count);
  * No reference counterpart (the reference has no timers on this path). */
 int nhf_bench_hpsi(void *ctx, int pid, int q, int m, int prec, int reps, double *avg_ms);
 
+/* Measurement: average duration (ms, HIP events on the species stream) of
+ * `reps` back-to-back block products A^H B of device-resident state blocks
+ * (ka and kb columns of the problem's work buffers, the problem's
+ * precision; lower = 1: the Hermitian A^H A with ka = kb), including the
+ * reduction over point chunks.  No counterpart in the reference. */
+int nhf_bench_gram(void *ctx, int pid, int q, int ka, int kb, int lower, int reps, double *avg_ms);
+
 /* Measurement: live timing (HIP events on the species streams) of the
  * kernels of the SCF steps.  enable = 1 starts (and resets) the timing,
  * 0 stops it, -1 leaves it.  out (may be NULL), per kernel class k

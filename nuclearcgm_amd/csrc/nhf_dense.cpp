@@ -16,6 +16,11 @@
 
 namespace nhf {
 
+// sqrt(a^2 + b^2) without the over/underflow guards of hypot (a library
+// call that dominated the QL sweep): the projected matrices have entries of
+// at most ~1e4 MeV
+static inline double pyth(double a, double b) { return std::sqrt(a * a + b * b); }
+
 // implicit QL with Wilkinson shifts on the symmetric tridiagonal (d, e),
 // e[i] coupling i and i+1, eigenvalues only (returned in d, unsorted)
 static bool tql_values(int k, double* d, double* e) {
@@ -30,13 +35,13 @@ static bool tql_values(int k, double* d, double* e) {
       if (mm != l) {
         if (iter++ == 60) return false;
         double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-        double r = std::hypot(g, 1.0);
+        double r = pyth(g, 1.0);
         g = d[mm] - d[l] + e[l] / (g + std::copysign(r, g));
         double s = 1.0, c = 1.0, p = 0.0;
         int i;
         for (i = mm - 1; i >= l; --i) {
           const double f = s * e[i], b = c * e[i];
-          e[i + 1] = (r = std::hypot(f, g));
+          e[i + 1] = (r = pyth(f, g));
           if (r == 0.0) {
             d[i + 1] -= p;
             e[mm] = 0.0;
@@ -150,8 +155,9 @@ static bool tri_inverse_iteration(int k, const double* d, const double* e, const
 bool eigh_lower(int k, const cd* Ain, double* w, cd* V, int nvec) {
   if (k <= 0) return false;
   if (nvec < 0 || nvec > k) nvec = k;
-  // full Hermitian matrix from the lower triangle, split real / imaginary
-  // parts (column-major) so that the O(k^3) loops vectorise
+  // the lower triangle, split into real / imaginary parts (column-major) so
+  // that the O(k^3) loops vectorise; the reduction reads and updates the
+  // lower triangle only
   std::vector<double> Ar((size_t)k * k), Ai((size_t)k * k);
   for (int c = 0; c < k; ++c)
     for (int r = 0; r < k; ++r) {
@@ -193,17 +199,24 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V, int nvec) {
     }
     e[i] = alpha.real();
     if (ti != cd(0.0, 0.0)) {
-      // y = A22 v (column sweep), x = tau y
+      // y = A22 v from the lower triangle of A22 (the upper one is its
+      // conjugate), x = tau y
       std::fill(xr.begin(), xr.begin() + m, 0.0);
       std::fill(xi.begin(), xi.begin() + m, 0.0);
       for (int c = 0; c < m; ++c) {
         const double* ar = &Ar[(size_t)(o + c) * k + o];
         const double* ai = &Ai[(size_t)(o + c) * k + o];
         const double br = vr[c], bi = vi[c];
-        for (int r = 0; r < m; ++r) {
+        double sr = ar[c] * br, si = ar[c] * bi;  // real diagonal
+        for (int r = c + 1; r < m; ++r) {
           xr[r] += ar[r] * br - ai[r] * bi;
           xi[r] += ar[r] * bi + ai[r] * br;
+          // conj(A[r, c]) v[r]
+          sr += ar[r] * vr[r] + ai[r] * vi[r];
+          si += ar[r] * vi[r] - ai[r] * vr[r];
         }
+        xr[c] += sr;
+        xi[c] += si;
       }
       double xvr = 0.0, xvi = 0.0;  // (tau y)^H v
       for (int r = 0; r < m; ++r) {
@@ -218,12 +231,12 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V, int nvec) {
         xr[r] += al2.real() * vr[r] - al2.imag() * vi[r];
         xi[r] += al2.real() * vi[r] + al2.imag() * vr[r];
       }
-      // A22 -= v x^H + x v^H
+      // A22 -= v x^H + x v^H, lower triangle
       for (int c = 0; c < m; ++c) {
         double* ar = &Ar[(size_t)(o + c) * k + o];
         double* ai = &Ai[(size_t)(o + c) * k + o];
         const double xcr = xr[c], xci = -xi[c], vcr = vr[c], vci = -vi[c];  // conjugates
-        for (int r = 0; r < m; ++r) {
+        for (int r = c; r < m; ++r) {
           ar[r] -= vr[r] * xcr - vi[r] * xci + xr[r] * vcr - xi[r] * vci;
           ai[r] -= vr[r] * xci + vi[r] * xcr + xr[r] * vci + xi[r] * vcr;
         }

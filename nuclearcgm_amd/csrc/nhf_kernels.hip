@@ -21,22 +21,12 @@ namespace nhf {
 
 constexpr int BS = 256;
 
-// NHF_GRAM=v1: the previous split-BF16 Gram kernel (one 32 x 32 block per
-// workgroup, 16x16x16 MFMA), kept for comparison
+// NHF_GRAM=v1: the 32 x 32 block split-BF16 kernel (16x16x16 MFMA) for every
+// shape of FP32 block product, not only the small ones
 static bool gram_v1() {
   static bool v = [] {
     const char* e = std::getenv("NHF_GRAM");
     return e && std::strcmp(e, "v1") == 0;
-  }();
-  return v;
-}
-// workgroups the LDS-staged Gram aims for (NHF_GRAM_WG, default 256: one
-// 12-wavefront workgroup per CU)
-static int gram_wg_target() {
-  static int v = [] {
-    const char* e = std::getenv("NHF_GRAM_WG");
-    const int x = e ? std::atoi(e) : 0;
-    return x > 0 ? x : 256;
   }();
   return v;
 }
@@ -805,8 +795,8 @@ __global__ __launch_bounds__(BS) void k_gram_mfma(const Sp<float>* __restrict__ 
 
 // ---- LDS-staged split-BF16 block product of FP32 states
 // (v_mfma_f32_16x16x32_bf16).  A workgroup owns a GS x GS output super-tile
-// (rows I0.., columns J0..) over one chunk of points, so each column of A and
-// B is read from HBM once per chunk.  Per slab of 8 points (K = 32 real
+// (rows I0.., columns J0..) over one chunk of GCHUNK points, so each column
+// of A and B is read from HBM once per chunk.  Per slab of 8 points (K = 32 real
 // components) the workgroup splits the super-tile's columns exactly into
 // three BF16 parts, x = h + m + l (split2), and stages them in LDS; each
 // wavefront then forms the six products
@@ -819,9 +809,9 @@ __global__ __launch_bounds__(BS) void k_gram_mfma(const Sp<float>* __restrict__ 
 // columns of the 6 x 6 tiles of a super-tile (rows w / 2, columns
 // 3 (w % 2)..); in the diagonal super-tile of a Hermitian product only the
 // lower tile triangle, as nine row triples.  FP32 accumulators are flushed
-// into FP64 sums every two slabs; the FP64 chunk partials are summed by
-// k_reduce_parts (FP32 partials would add a rounding of each chunk sum,
-// 6e-8 of it, which the sum over few chunks does not average out).
+// into FP64 sums every two slabs; the chunk results are stored as FP32
+// partials (a rounding of 2^-24 of each chunk sum, independent across
+// chunks) and summed in FP64 by k_reduce_parts_f.
 typedef __bf16 gbf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int gu32x4 __attribute__((ext_vector_type(4)));
 constexpr int GS = 96;   // super-tile edge
@@ -829,6 +819,10 @@ constexpr int GSU = 5;   // LDS units per column and split level (4 + 1 pad)
 constexpr int GUNITS = 2 * 3 * GS * GSU;
 constexpr int GWAVES = 12;
 constexpr int GBS = 64 * GWAVES;
+constexpr int GPF = 4;      // slabs of operands in flight per thread
+constexpr int GSLABS = 32;  // slabs (8 points) per workgroup
+constexpr int GUNROLL = 8;  // slabs per loop iteration (code size vs. waits at the loop head)
+constexpr int GCHUNK = 8 * GSLABS;
 typedef __bf16 gbf16x2 __attribute__((ext_vector_type(2)));
 typedef float gf2v __attribute__((ext_vector_type(2)));
 // two FP32 values rounded to nearest-even BF16, packed (v_cvt_pk_bf16_f32)
@@ -876,7 +870,7 @@ __device__ __forceinline__ void tile12(gf32x4& re, gf32x4& im, const gu32x4 (&a)
 }
 __global__ __launch_bounds__(GBS) void k_gram_lds(const Sp<float>* __restrict__ A, size_t lda, int ka,
                                                   const Sp<float>* __restrict__ B, size_t ldb, int kb, size_t N3,
-                                                  int chunk_pts, double2* __restrict__ part, int lower) {
+                                                  int chunk_pts, float2* __restrict__ part, int lower) {
   __shared__ gu32x4 lds[GUNITS];
   const int nsj = (kb + GS - 1) / GS;
   const int I0 = (blockIdx.x / nsj) * GS, J0 = (blockIdx.x % nsj) * GS;
@@ -885,8 +879,10 @@ __global__ __launch_bounds__(GBS) void k_gram_lds(const Sp<float>* __restrict__ 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, c = lane & 15;
   const size_t p0 = (size_t)blockIdx.y * chunk_pts;
   const size_t p1 = min(N3, p0 + (size_t)chunk_pts);
-  // staging units: (matrix, column, group); the diagonal super-tile needs A only
-  const int nunits = (diag ? 1 : 2) * GS * 4;
+  // staging items (matrix, column, point of the slab), point fastest: eight
+  // threads read one column's 128-byte line; the diagonal super-tile needs
+  // A only
+  const int nitems = (diag ? 1 : 2) * GS * 8;
   // this wavefront's tile row and first tile column
   int trow, tc0;
   bool act;
@@ -914,72 +910,100 @@ __global__ __launch_bounds__(GBS) void k_gram_lds(const Sp<float>* __restrict__ 
 #pragma unroll
     for (int r = 0; r < 4; ++r) dre[s][r] = dim[s][r] = 0.0;
   }
-  // register prefetch of one slab: one unit per thread, two points
-  float4 pf[2];
-  const bool has_unit = tid < nunits;
-  const int umat = tid / (GS * 4), ucol = (tid >> 2) % GS, ugrp = tid & 3;
-  const int ugc = (umat ? J0 : I0) + ucol;
-  const bool ucol_ok = has_unit && ugc < (umat ? kb : ka);
-  const float4* usrc = reinterpret_cast<const float4*>((umat ? B : A) + (size_t)(ucol_ok ? ugc : 0) * (umat ? ldb : lda));
-  auto fetch = [&](size_t base) {
-    pf[0] = pf[1] = make_float4(0.f, 0.f, 0.f, 0.f);
-    const size_t p = base + 2 * ugrp;
-    if (ucol_ok) {
-      if (p < p1) pf[0] = usrc[p];
-      if (p + 1 < p1) pf[1] = usrc[p + 1];
-    }
+  // register prefetch ring of GPF slabs, up to two items (points) per thread
+  float4 pf[GPF][2];
+  const float4* isrc[2];
+  bool iok[2], ihas[2];
+  int idst[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int i = tid + k * GBS;
+    ihas[k] = i < nitems;
+    const int mat = i / (GS * 8), col = (i >> 3) % GS, pt = i & 7;
+    const int gc = (mat ? J0 : I0) + col;
+    iok[k] = ihas[k] && gc < (mat ? kb : ka);
+    isrc[k] = reinterpret_cast<const float4*>((mat ? B : A) + (size_t)(iok[k] ? gc : 0) * (mat ? ldb : lda));
+    // unit (matrix, level 0, column, group pt / 2), half pt % 2, in 8-byte words
+    idst[k] = 2 * (((mat * 3) * GS + col) * GSU + (pt >> 1)) + (pt & 1);
+  }
+  // loads are unconditional (clamped addresses, masked after arrival), so
+  // that no branch around them makes the compiler wait for every load in
+  // flight
+  const size_t plast = p1 - 1;
+  auto fetch = [&](float4 (&dst)[2], size_t base) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) dst[k] = isrc[k][min(base + (size_t)(tid & 7), plast)];
   };
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  unsigned* ldsw = reinterpret_cast<unsigned*>(lds);
   const int bmat = diag ? 0 : 1;
   const int ua = (16 * trow + c) * GSU + g;
   const int ub0 = ((bmat * 3) * GS + 16 * tc0 + c) * GSU + g;
-  fetch(p0);
-  int ns = 0;
-  for (size_t base = p0; base < p1; base += 8) {
-    __syncthreads();  // the previous slab's operands are consumed
-    if (has_unit) {
-      unsigned hh[4], mm[4], ll[4];
-      split2(pf[0].x, pf[0].y, hh[0], mm[0], ll[0]);
-      split2(pf[0].z, pf[0].w, hh[1], mm[1], ll[1]);
-      split2(pf[1].x, pf[1].y, hh[2], mm[2], ll[2]);
-      split2(pf[1].z, pf[1].w, hh[3], mm[3], ll[3]);
-      const int ud = ((umat * 3) * GS + ucol) * GSU + ugrp;
-      lds[ud] = gu32x4{hh[0], hh[1], hh[2], hh[3]};
-      lds[ud + GS * GSU] = gu32x4{mm[0], mm[1], mm[2], mm[3]};
-      lds[ud + 2 * GS * GSU] = gu32x4{ll[0], ll[1], ll[2], ll[3]};
-    }
-    if (base + 8 < p1) fetch(base + 8);
-    __syncthreads();
-    if (act) {
-      gu32x4 a[3];
+  // the GCHUNK points of the workgroup as GSLABS slabs, GUNROLL per loop
+  // iteration, with a GPF-deep register prefetch ring: within an iteration
+  // the compiler counts the loads in flight exactly (at the loop head it
+  // waits for all of them); a fully unrolled chunk overflows the
+  // instruction cache
 #pragma unroll
-      for (int v = 0; v < 3; ++v) a[v] = lds[ua + v * GS * GSU];
+  for (int d = 0; d < GPF; ++d) fetch(pf[d], p0 + 8 * d);
+  for (int sg = 0; sg < GSLABS; sg += GUNROLL) {
 #pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        if (!on[s]) continue;
-        gu32x4 b[3], bx[3];
+  for (int su = 0; su < GUNROLL; ++su) {
+    const int sl = sg + su;
+    const int d = su % GPF;
+    const size_t base = p0 + 8 * sl;
+    if (base < p1) {
+      __syncthreads();  // the previous slab's operands are consumed
 #pragma unroll
-        for (int v = 0; v < 3; ++v) {
-          b[v] = lds[ub0 + 16 * s * GSU + v * GS * GSU];
-          bx[v] = mul_mi8(b[v]);
-        }
-        tile12(re[s], im[s], a, b, bx);
+      for (int k = 0; k < 2; ++k) {
+        if (!ihas[k]) continue;
+        const float4 v = iok[k] && base + (tid & 7) < p1 ? pf[d][k] : zero4;
+        unsigned h0, m0, l0, h1, m1, l1;
+        split2(v.x, v.y, h0, m0, l0);
+        split2(v.z, v.w, h1, m1, l1);
+        unsigned* w = ldsw + 2 * idst[k];
+        *reinterpret_cast<uint2*>(w) = make_uint2(h0, h1);
+        *reinterpret_cast<uint2*>(w + 4 * GS * GSU) = make_uint2(m0, m1);
+        *reinterpret_cast<uint2*>(w + 8 * GS * GSU) = make_uint2(l0, l1);
       }
     }
-    if (++ns == 2) {
-      ns = 0;
+    fetch(pf[d], p0 + 8 * (sl + GPF));  // past the chunk: clamped, unused
+    if (base < p1) {
+      __syncthreads();
+      if (act) {
+        gu32x4 a[3];
 #pragma unroll
-      for (int s = 0; s < 3; ++s) {
+        for (int v = 0; v < 3; ++v) a[v] = lds[ua + v * GS * GSU];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          dre[s][r] += re[s][r];
-          dim[s][r] += im[s][r];
+        for (int s = 0; s < 3; ++s) {
+          if (!on[s]) continue;
+          gu32x4 b[3], bx[3];
+#pragma unroll
+          for (int v = 0; v < 3; ++v) {
+            b[v] = lds[ub0 + 16 * s * GSU + v * GS * GSU];
+            bx[v] = mul_mi8(b[v]);
+          }
+          tile12(re[s], im[s], a, b, bx);
         }
-        re[s] = gf32x4{0.f, 0.f, 0.f, 0.f};
-        im[s] = gf32x4{0.f, 0.f, 0.f, 0.f};
+      }
+      if (su & 1) {  // FP64 flush every two slabs
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            dre[s][r] += re[s][r];
+            dim[s][r] += im[s][r];
+          }
+          re[s] = gf32x4{0.f, 0.f, 0.f, 0.f};
+          im[s] = gf32x4{0.f, 0.f, 0.f, 0.f};
+        }
       }
     }
   }
-  double2* out = part + (size_t)blockIdx.y * ka * kb;
+  }
+  // a partial last chunk may end on an odd slab: the write below adds the
+  // unflushed FP32 part
+  float2* out = part + (size_t)blockIdx.y * ka * kb;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     if (!on[s]) continue;
@@ -987,10 +1011,51 @@ __global__ __launch_bounds__(GBS) void k_gram_lds(const Sp<float>* __restrict__ 
     for (int r = 0; r < 4; ++r) {
       const int i = I0 + 16 * trow + 4 * g + r, j = J0 + 16 * (tc0 + s) + c;
       if (i < ka && j < kb)
-        out[(size_t)j * ka + i] = make_double2(dre[s][r] + re[s][r], dim[s][r] + im[s][r]);
+        out[(size_t)j * ka + i] = make_float2((float)(dre[s][r] + re[s][r]), (float)(dim[s][r] + im[s][r]));
     }
   }
 }
+// sums FP32 chunk partials in FP64 (four independent chains per thread);
+// entries (i, j) with i / edge < j / edge (edge > 0: Hermitian product,
+// upper tiles not computed) are the conjugate of their mirror
+__global__ __launch_bounds__(BS) void k_reduce_parts_f(const float2* __restrict__ part, int nchunk, size_t cnt,
+                                                       int ka, int edge, double2* __restrict__ C) {
+  __shared__ double2 sh[BS / 32][32];
+  const int le = threadIdx.x % 32, cg = threadIdx.x / 32;
+  const size_t e = (size_t)blockIdx.x * 32 + le;
+  double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+  bool mirror = false;
+  if (e < cnt) {
+    const int i = (int)(e % ka), j = (int)(e / ka);
+    mirror = edge > 0 && (i / edge) < (j / edge);
+    const size_t src = mirror ? (size_t)i * ka + j : e;
+    constexpr int G = BS / 32;
+    int ch = cg;
+    for (; ch + 3 * G < nchunk; ch += 4 * G) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float2 v = part[(size_t)(ch + u * G) * cnt + src];
+        sr[u] += v.x;
+        si[u] += v.y;
+      }
+    }
+    for (; ch < nchunk; ch += G) {
+      const float2 v = part[(size_t)ch * cnt + src];
+      sr[0] += v.x;
+      si[0] += v.y;
+    }
+  }
+  sh[cg][le] = make_double2((sr[0] + sr[1]) + (sr[2] + sr[3]), (si[0] + si[1]) + (si[2] + si[3]));
+  __syncthreads();
+  if (cg != 0 || e >= cnt) return;
+  double tr = sh[0][le].x, ti = sh[0][le].y;
+  for (int k = 1; k < BS / 32; ++k) {
+    tr += sh[k][le].x;
+    ti += sh[k][le].y;
+  }
+  C[e] = mirror ? make_double2(tr, -ti) : make_double2(tr, ti);
+}
+
 // chunk length (points) for the split Gram: enough workgroups to fill the
 // device (about 1024), a multiple of 16, at least 256 points
 static int gram_mfma_chunk(size_t N3, int blocks) {
@@ -1005,21 +1070,19 @@ void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>
   lower = (lower && A == B && ka == kb) ? 1 : 0;
   size_t cnt = (size_t)ka * kb;
   hipEvent_t e0 = ev ? ev->start : nullptr, e1 = ev ? ev->stop : nullptr, none = nullptr;
-  if (std::is_same<R, float>::value && !block_fp64() && !gram_v1()) {
+  // the super-tile kernel reads each column once per chunk but runs at the
+  // pace of a 96 x 96 product; the 32 x 32 block kernel is faster for small
+  // products (measured on the Ca-40 shapes: 32 x 32 34 vs 52 us, 96 x 32 50
+  // vs 60 us, 96 x 96 general 138 vs 81 us)
+  const bool big = (size_t)ka * kb >= 64 * 64;
+  if (std::is_same<R, float>::value && !block_fp64() && !gram_v1() && big) {
     const int nsi = (ka + GS - 1) / GS, nsj = (kb + GS - 1) / GS;
-    const int active = lower ? nsi * (nsi + 1) / 2 : nsi * nsj;
-    const size_t want = (size_t)std::max(1, gram_wg_target() / active);
-    size_t cp = (N3 + want - 1) / want;
-    cp = (cp + 7) / 8 * 8;
-    size_t nc = (N3 + cp - 1) / cp;
-    if (nc > (size_t)nchunk) {  // partial-sum slots of the caller's buffer
-      nc = (size_t)nchunk;
-      cp = ((N3 + nc - 1) / nc + 7) / 8 * 8;
-      nc = (N3 + cp - 1) / cp;
-    }
+    // fixed chunks of GCHUNK points: `part` holds gram_chunks(N3) chunk slots
+    const size_t cp = GCHUNK, nc = (N3 + cp - 1) / cp;
+    float2* pf = reinterpret_cast<float2*>(part);
     LAUNCH_EV(e0, none, k_gram_lds, dim3(nsi * nsj, (unsigned)nc), dim3(GBS), 0, st, (const Sp<float>*)A, lda, ka,
-              (const Sp<float>*)B, ldb, kb, N3, (int)cp, part, lower);
-    LAUNCH_EV(none, e1, k_reduce_parts, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, (int)nc, cnt, ka,
+              (const Sp<float>*)B, ldb, kb, N3, (int)cp, pf, lower);
+    LAUNCH_EV(none, e1, k_reduce_parts_f, dim3(reduce_grid(cnt)), dim3(BS), 0, st, pf, (int)nc, cnt, ka,
               lower ? 16 : 0, C);
     return;
   }

@@ -1900,6 +1900,40 @@ int nhf_bench_hpsi(void* ctx, int pid, int q, int m, int prec, int reps, double*
   return NHF_OK;
 }
 
+int nhf_bench_gram(void* ctx, int pid, int q, int ka, int kb, int lower, int reps, double* avg_ms) {
+  Problem* P = get_problem(ctx, pid);
+  if (!P || q < 0 || q > 1 || ka < 1 || kb < 1 || ka > P->kmax || kb > P->kmax || reps < 1 || !avg_ms ||
+      (lower && ka != kb))
+    return NHF_EINVAL;
+  Work& W = P->W[q];
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  // operands: constant-filled work buffers (timing only)
+  const size_t bytes = (P->prec == NHF_PREC_FP64 ? sizeof(Sp<double>) : sizeof(Sp<float>)) * P->N3 * P->kmax;
+  HIPCHK(hipMemsetAsync(W.V, 0, bytes, W.st));
+  HIPCHK(hipMemsetAsync(W.AV, 0, bytes, W.st));
+  auto run = [&]() {
+    if (P->prec == NHF_PREC_FP64)
+      launch_gram<double>(W.st, (const Sp<double>*)W.V, P->N3, ka, lower ? (const Sp<double>*)W.V : (const Sp<double>*)W.AV,
+                          P->N3, kb, P->N3, W.gpart, P->gchunks, W.gout, lower);
+    else
+      launch_gram<float>(W.st, (const Sp<float>*)W.V, P->N3, ka, lower ? (const Sp<float>*)W.V : (const Sp<float>*)W.AV,
+                         P->N3, kb, P->N3, W.gpart, P->gchunks, W.gout, lower);
+  };
+  run();
+  HIPCHK(hipEventRecord(e0, W.st));
+  for (int r = 0; r < reps; ++r) run();
+  HIPCHK(hipEventRecord(e1, W.st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = ms / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return NHF_OK;
+}
+
 int nhf_kernel_stats(void* ctx, int pid, int enable, double* out) {
   Problem* P = get_problem(ctx, pid);
   if (!P) return NHF_EINVAL;

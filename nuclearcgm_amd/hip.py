@@ -47,6 +47,8 @@ def lib():
         L.nhf_batch_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(NhfBatchItem), ctypes.c_int]
         L.nhf_bench_hpsi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _D]
+        L.nhf_bench_gram.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, _D]
         L.nhf_kernel_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         L.nhf_phase_times.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]
         L.nhf_ho_guess.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]
@@ -354,6 +356,12 @@ class Problem:
         ms = ctypes.c_double()
         _chk(lib().nhf_bench_hpsi(self.ctx.h, self.pid, q, m, self.cfg.precision if prec is None else prec, reps,
                                   ctypes.byref(ms)))
+        return ms.value
+
+    def bench_gram(self, q, ka, kb, lower=False, reps=50):
+        """Average ms of one device block product A^H B (ka x kb columns)."""
+        ms = ctypes.c_double()
+        _chk(lib().nhf_bench_gram(self.ctx.h, self.pid, q, ka, kb, int(lower), reps, ctypes.byref(ms)))
         return ms.value
 
     def kernel_stats(self, enable=-1):

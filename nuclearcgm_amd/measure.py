@@ -13,6 +13,9 @@ hardware counters):
     python -m nuclearcgm_amd.measure stencil [--cols M] [--reps R]
         R launches of the H.psi stencil on M device-resident FP32 columns of
         the Ca-40 lattice
+    python -m nuclearcgm_amd.measure gram [--reps R]
+        R block products of device-resident FP32 state blocks of the Ca-40
+        lattice for the shapes of the GCG pass (k x k Hermitian, k x k, ...)
 """
 import argparse
 import os
@@ -23,7 +26,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["solve", "copy", "stencil"])
+    ap.add_argument("what", choices=["solve", "copy", "stencil", "gram"])
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--wstep-iters", type=int, default=1)
     ap.add_argument("--no-train", action="store_true")
@@ -61,6 +64,18 @@ def main(argv=None):
             p.set_fields(q, f, gf, U, W)
         ms = p.bench_hpsi(0, a.cols, C.PREC_MIXED, a.reps)
         print(f"stencil {a.cols} columns: {ms * 1e3:.2f} us per launch", flush=True)
+        return
+    if a.what == "gram":
+        cfg = C.workload("ca40", precision=C.PREC_MIXED)
+        p = ctx.problem(cfg)
+        M = cfg.M_n
+        n3 = cfg.n ** 3
+        for ka, kb, low in ((3 * M, 3 * M, True), (3 * M, 3 * M, False), (M, M, True), (3 * M, M, False),
+                            (2 * M, 2 * M, True)):
+            ms = p.bench_gram(0, ka, kb, low, a.reps)
+            mb = (ka + (0 if low else kb)) * n3 * 16 / 1e6
+            print(f"gram {ka}x{kb} {'hermitian' if low else 'general'}: {ms * 1e3:.2f} us per product, "
+                  f"{mb:.1f} MB algorithmic, {mb / ms / 1e3:.0f} GB/s", flush=True)
         return
     if a.no_train:
         cfg = bench.accelerated_config("ca40", C.W_JACOBI_CG, 0, C.MIX_BROYDEN, 8)

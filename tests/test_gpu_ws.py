@@ -38,7 +38,7 @@ def _subspace_cos(X, Y):
 
 
 @pytest.mark.parametrize("A,Z,M_n,M_p,b2,b3", [(16, 8, 8, 8, 0.0, 0.0), (20, 10, 12, 10, 0.3, 0.0),
-                                               (24, 12, 10, 12, 0.2, 0.08)])
+                                               (22, 12, 10, 12, 0.2, 0.08)])
 def test_ws_guess_matches_oracle(ctx, A, Z, M_n, M_p, b2, b3):
     """FP64 device start-up vs the oracle (same oscillator guess, same GCG).
 
