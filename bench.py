@@ -130,6 +130,7 @@ def main():
     ap.add_argument("--workload", default="ca40")
     ap.add_argument("--wstep-iters", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the ablation solves (profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the whole solve runs on a single MI355X; --gpus above 1 is not "
@@ -172,7 +173,8 @@ def main():
     d = ks[dom]
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
     names = {"stencil": "k_hpsi25<float> (H.psi stencil, FP32)",
-             "block": "k_gram_mfma (split-BF16 MFMA Gram / projection of FP32 states)"}
+             "block": "block products of FP32 states (split-BF16 MFMA: k_gram_lds 96x96 super-tiles for "
+                      "ka*kb >= 4096, k_gram_mfma 32x32 blocks below; + chunk reduction)"}
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{dom}.json")
     if os.path.exists(tf):
@@ -211,15 +213,17 @@ def main():
         "speedup_vs_plain_fp64": t64 / value if value > 0 else None,
     }
     # ablations (one solve each): the same path without the learned net
-    # (Jacobi-PCG W-block, 8 steps), and mixed precision alone
-    line["ablation"] = {
-        "mixed+broyden+jacobi_pcg8": solve(ctx, accelerated_config(args.workload, C.W_JACOBI_CG, 0, C.MIX_BROYDEN,
-                                                                   8), guess, e_ref),
-        "mixed+broyden+jacobi_pcg1": solve(ctx, accelerated_config(args.workload, C.W_JACOBI_CG, 0, C.MIX_BROYDEN,
-                                                                   1), guess, e_ref),
-        "mixed_only": solve(ctx, accelerated_config(args.workload, C.W_JACOBI_CG, 0, C.MIX_LINEAR, 80), guess,
-                            e_ref),
-    }
+    # (Jacobi-PCG W-block, 8 steps, and 1 step: the learned net's budget),
+    # and mixed precision alone
+    if not args.no_ablation:
+        line["ablation"] = {
+            "mixed+broyden+jacobi_pcg8": solve(ctx, accelerated_config(args.workload, C.W_JACOBI_CG, 0,
+                                                                       C.MIX_BROYDEN, 8), guess, e_ref),
+            "mixed+broyden+jacobi_pcg1": solve(ctx, accelerated_config(args.workload, C.W_JACOBI_CG, 0,
+                                                                       C.MIX_BROYDEN, 1), guess, e_ref),
+            "mixed_only": solve(ctx, accelerated_config(args.workload, C.W_JACOBI_CG, 0, C.MIX_LINEAR, 80), guess,
+                                e_ref),
+        }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg64, guess, it64)
     ctx.close()
