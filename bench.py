@@ -172,7 +172,7 @@ def main():
     dom = max(ks, key=lambda k: ks[k]["ms"])
     d = ks[dom]
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
-    names = {"stencil": "k_hpsi25<float> (H.psi stencil, FP32)",
+    names = {"stencil": "k_hpsi_row<float> (H.psi stencil, FP32)",
              "block": "block products of FP32 states (split-BF16 MFMA: k_gram_lds 96x96 super-tiles for "
                       "ka*kb >= 4096, k_gram_mfma 32x32 blocks below; + chunk reduction)"}
     traffic = None

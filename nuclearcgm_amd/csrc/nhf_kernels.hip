@@ -40,13 +40,13 @@ static bool block_fp64() {
   return v;
 }
 
-// 2.5D stencil launch shape: target workgroup count (environment
-// NHF_HPSI_WG, default 2048) and the XCD-aware block order (NHF_HPSI_XCD,
-// default on)
+// 2.5D stencil launch shape: the least workgroup count for which a longer
+// z chunk is chosen (environment NHF_HPSI_WG, default 768 = three per CU)
+// and the XCD-aware block order (NHF_HPSI_XCD, default on)
 static int hpsi_wg_target() {
   static int v = [] {
     const char* e = getenv("NHF_HPSI_WG");
-    return e ? std::max(1, atoi(e)) : 2048;
+    return e ? std::max(1, atoi(e)) : 768;
   }();
   return v;
 }
@@ -58,7 +58,7 @@ static int hpsi_xcd() {
   return v;
 }
 
-// stencil variant: 1 = 2.5D LDS/register kernel (default), 0 = direct loads
+// stencil variant: 1 = row-tile 2.5D LDS/register kernel (default), 0 = direct loads
 // (environment NHF_HPSI=direct)
 static int hpsi_variant() {
   static int v = [] {
@@ -212,21 +212,99 @@ __global__ __launch_bounds__(BS) void k_hpsi(const Sp<R>* __restrict__ in, Sp<R>
   if (DOT) block_atomic_add<2>(acc, dot + (size_t)dot_stride * c);
 }
 
-// 2.5D variant: a workgroup owns TP = 256 consecutive points q = x + n y of
-// an x-y plane and marches over a chunk of z planes.  The point's own z
-// neighbours live in a 5-deep register queue, the plane tile plus a halo of
-// two rows on each side is staged in LDS, so each spinor point is read from
-// global memory ~1 + 4n/TP times instead of 13.
-template <typename R, bool DOT>
-__global__ __launch_bounds__(BS) void k_hpsi25(const Sp<R>* __restrict__ in, Sp<R>* __restrict__ out, size_t ld,
-                                               const R* __restrict__ fld, int n, R c1, R c2, R shift, int zc,
-                                               double* __restrict__ dot, int dot_stride,
-                                               const int* __restrict__ done, unsigned long long* __restrict__ act,
-                                               const int* __restrict__ clist, int ntile, int nzc, int ncols,
-                                               int xcd) {
-  extern __shared__ char smem[];
-  Sp<R>* tile = reinterpret_cast<Sp<R>*>(smem);  // BS + 4n entries
-  // logical block L = (column, z chunk, plane tile), tile fastest.  With
+// Row-tile 2.5D variant.  A workgroup owns `rt` whole x rows of an x-y
+// plane (rt n <= 256 threads, one point each) and marches over a chunk of
+// ZC z planes (compile-time, so the loop unrolls and every load's wait is
+// placed exactly).  The point's z neighbours live in a 5-deep register
+// queue; the rows plus two halo rows on each side are staged in a
+// zero-padded LDS image of (rt + 4) x (n + 4) points, so no neighbour read
+// needs a boundary test (the pads and the halo rows outside the box stay
+// zero: the stencil is truncated at the box).  Two LDS images alternate
+// between planes (one barrier per plane).  The loads of plane z + 3, of the
+// halo rows and of the fields of plane z + 1 are issued one plane ahead.
+// The field scalars are scaled by the stencil denominators once per point
+// (f / 12h^2, grad f / 12h, W / 12h), so the derivative sums stay unscaled.
+// a spinor point loaded when `ok`, else zero: the choice is made on the
+// loaded value (a select between a global address and a stack temporary
+// would force the temporary into scratch and the load into a flat load)
+template <typename R>
+__device__ __forceinline__ Sp<R> ld_sp(const Sp<R>* __restrict__ p, bool ok) {
+  typedef R v4 __attribute__((ext_vector_type(4)));
+  v4 v = {R(0), R(0), R(0), R(0)};
+  if (ok) v = *reinterpret_cast<const v4*>(p);
+  return Sp<R>{v.x, v.y, v.z, v.w};
+}
+
+template <typename R>
+__device__ __forceinline__ void nt_store(Sp<R>* p, const Sp<R>& o) {
+  typedef R v4 __attribute__((ext_vector_type(4)));
+  const v4 v = {o.ur, o.ui, o.dr, o.di};
+  __builtin_nontemporal_store(v, reinterpret_cast<v4*>(p));
+}
+
+template <typename R>
+__device__ __forceinline__ Sp<R> hpsi_point(Sp<R> zm2, Sp<R> zm1, Sp<R> a0, Sp<R> zp1, Sp<R> zp2,
+                                            const Sp<R>* __restrict__ t, int s, int W, const R (&F)[NFLD]) {
+  const Sp<R> xm1 = t[s - 1], xm2 = t[s - 2], xp1 = t[s + 1], xp2 = t[s + 2];
+  const Sp<R> ym1 = t[s - W], ym2 = t[s - 2 * W], yp1 = t[s + W], yp2 = t[s + 2 * W];
+  Sp<R> d[3];
+#define NHF_D(dd, P1, M1, P2, M2)                     \
+  dd.ur = R(8) * (P1.ur - M1.ur) - (P2.ur - M2.ur);   \
+  dd.ui = R(8) * (P1.ui - M1.ui) - (P2.ui - M2.ui);   \
+  dd.dr = R(8) * (P1.dr - M1.dr) - (P2.dr - M2.dr);   \
+  dd.di = R(8) * (P1.di - M1.di) - (P2.di - M2.di);
+  NHF_D(d[0], xp1, xm1, xp2, xm2)
+  NHF_D(d[1], yp1, ym1, yp2, ym2)
+  NHF_D(d[2], zp1, zm1, zp2, zm2)
+#undef NHF_D
+#define NHF_L(c)                                                                                         \
+  (R(16) * ((xp1.c + xm1.c) + (yp1.c + ym1.c) + (zp1.c + zm1.c)) -                                     \
+   ((xp2.c + xm2.c) + (yp2.c + ym2.c) + (zp2.c + zm2.c)) - R(90) * a0.c)
+  const Sp<R> lap = {NHF_L(ur), NHF_L(ui), NHF_L(dr), NHF_L(di)};
+#undef NHF_L
+  const R f = F[0], gx = F[1], gy = F[2], gz = F[3], U = F[4], Wx = F[5], Wy = F[6], Wz = F[7];
+  // T_0 = -Wy dx0 - i Wz dx1 + Wx dy0 - Wz dy1 + (Wy + i Wx) dz1
+  // T_1 = i Wz dx0 + Wy dx1 - Wz dy0 - Wx dy1 + (Wy - i Wx) dz0
+  const R t0r = -Wy * d[0].ur + Wz * d[0].di + Wx * d[1].ur - Wz * d[1].dr + Wy * d[2].dr - Wx * d[2].di;
+  const R t0i = -Wy * d[0].ui - Wz * d[0].dr + Wx * d[1].ui - Wz * d[1].di + Wy * d[2].di + Wx * d[2].dr;
+  const R t1r = -Wz * d[0].ui + Wy * d[0].dr - Wz * d[1].ur - Wx * d[1].dr + Wy * d[2].ur + Wx * d[2].ui;
+  const R t1i = Wz * d[0].ur + Wy * d[0].di - Wz * d[1].ui - Wx * d[1].di + Wy * d[2].ui - Wx * d[2].ur;
+  // -i T = (T.im, -T.re)
+  Sp<R> o;
+  o.ur = -f * lap.ur - (gx * d[0].ur + gy * d[1].ur + gz * d[2].ur) + U * a0.ur + t0i;
+  o.ui = -f * lap.ui - (gx * d[0].ui + gy * d[1].ui + gz * d[2].ui) + U * a0.ui - t0r;
+  o.dr = -f * lap.dr - (gx * d[0].dr + gy * d[1].dr + gz * d[2].dr) + U * a0.dr + t1i;
+  o.di = -f * lap.di - (gx * d[0].di + gy * d[1].di + gz * d[2].di) + U * a0.di - t1r;
+  return o;
+}
+
+template <typename R>
+__device__ __forceinline__ void load_fields_scaled(const R* __restrict__ fld, size_t p, R c1, R c2, R shift,
+                                                   R (&F)[NFLD]) {
+  load_fields(fld, p, F);
+  F[0] *= c2;
+  F[1] *= c1;
+  F[2] *= c1;
+  F[3] *= c1;
+  F[4] += shift;
+  F[5] *= c1;
+  F[6] *= c1;
+  F[7] *= c1;
+}
+
+constexpr int HPSI_MAXT = 256;  // threads per workgroup at most (rt n <= 256)
+
+template <typename R, bool DOT, int ZC>
+__global__ __launch_bounds__(HPSI_MAXT) void k_hpsi_row(const Sp<R>* __restrict__ in, Sp<R>* __restrict__ out,
+                                                        size_t ld, const R* __restrict__ fld, int n, int rt, R c1,
+                                                        R c2, R shift, double* __restrict__ dot, int dot_stride,
+                                                        const int* __restrict__ done,
+                                                        unsigned long long* __restrict__ act,
+                                                        const int* __restrict__ clist, int ntile, int nzc, int ncols,
+                                                        int xcd) {
+  extern __shared__ __align__(16) unsigned char hsm[];
+  Sp<R>* lds = reinterpret_cast<Sp<R>*>(hsm);
+  // logical block L = (column, z chunk, row tile), tile fastest.  With
   // xcd != 0 the workgroups that the dispatcher sends to one XCD (ids
   // w = xcd_index (mod 8)) get one contiguous range of L, so that tiles
   // sharing halo rows and chunks sharing halo planes meet in one L2.
@@ -241,111 +319,93 @@ __global__ __launch_bounds__(BS) void k_hpsi25(const Sp<R>* __restrict__ in, Sp<
   const int c = clist ? clist[bc] : bc;
   if (done && done[c]) return;  // uniform over the workgroup
   if (act && bt == 0 && bz == 0 && threadIdx.x == 0) atomicAdd(act, 1ull);
-  const int nn = n * n;
+  const int nn = n * n, W = n + 4, TS = (rt + 4) * W;
   const int tid = threadIdx.x;
-  const int q0 = bt * BS;
-  const int q = q0 + tid;
-  const bool own = q < nn;
-  const int x = own ? q % n : 0;
-  const int z0 = bz * zc;
-  const int z1 = min(n, z0 + zc);
-  const size_t N3 = (size_t)nn * n;
+  const Sp<R> Z = sp_zero<R>();
+  for (int i = tid; i < 2 * TS; i += blockDim.x) lds[i] = Z;
+  const int j0 = bt * rt;
+  const int rows = min(rt, n - j0);
+  const int r = tid / n, x = tid - r * n;
+  const bool own = r < rows;
+  const int q = own ? (j0 + r) * n + x : 0;
+  const int slot = (r + 2) * W + x + 2;
+  // halo rows j0-2, j0-1 (image rows 0, 1) and j0+rows, j0+rows+1 (image
+  // rows rows+2, rows+3); rows outside the box are never written
+  int hq = -1, hslot = 0;
+  if (tid < 4 * n) {
+    const int hr = tid / n, hx = tid - hr * n;
+    const int jj = hr < 2 ? j0 - 2 + hr : j0 + rows + hr - 2;
+    if (jj >= 0 && jj < n) {
+      hq = jj * n + hx;
+      hslot = (hr < 2 ? hr : rows + hr) * W + hx + 2;
+    }
+  }
+  const int z0 = bz * ZC;
   const Sp<R>* __restrict__ src = in + (size_t)c * ld;
   Sp<R>* __restrict__ dst = out + (size_t)c * ld;
-  const Sp<R> Z = sp_zero<R>();
-  auto ldz = [&](int z) -> Sp<R> { return (own && z >= 0 && z < n) ? src[(size_t)z * nn + q] : Z; };
-  Sp<R> zm2 = ldz(z0 - 2), zm1 = ldz(z0 - 1), z00 = ldz(z0), zp1 = ldz(z0 + 1);
-  // halo: entries [0, 2n) hold q0-2n .. q0-1, [2n+BS, 4n+BS) hold q0+BS .. q0+BS+2n-1
-  const int nh = 4 * n;
-  int hq = -1, hslot = -1;
-  if (tid < nh) {
-    if (tid < 2 * n) {
-      hq = q0 - 2 * n + tid;
-      hslot = tid;
-    } else {
-      hq = q0 + BS + (tid - 2 * n);
-      hslot = BS + tid;
-    }
-    if (hq < 0 || hq >= nn) hq = -1;
-  }
-  double acc[2] = {0.0, 0.0};
-  // software pipeline: the loads of step z+1 (plane z+3, the halo and the
-  // fields of plane z+1) are issued one step ahead, so their latency hides
-  // behind the current step
-  auto ldh = [&](int z) -> Sp<R> { return (hq >= 0 && z < n) ? src[(size_t)z * nn + hq] : Z; };
-  Sp<R> zp2 = ldz(z0 + 2), hcur = ldh(z0);
+  // the z queue: planes z-2 .. z+2 of the point, in five named registers
+  auto ldz = [&](int z) -> Sp<R> { return ld_sp(src + (size_t)z * nn + q, own && z >= 0 && z < n); };
+  Sp<R> zm2 = ldz(z0 - 2), zm1 = ldz(z0 - 1), z00 = ldz(z0), zp1 = ldz(z0 + 1), zp2 = ldz(z0 + 2);
+  Sp<R> hc = ld_sp(src + (size_t)z0 * nn + hq, hq >= 0);
+  // field loads are unconditional (threads without a point read point 0),
+  // so that F and its prefetch stay in registers
   R F[NFLD];
-  if (own) load_fields(fld, (size_t)z0 * nn + q, F);
-  for (int z = z0; z < z1; ++z) {
-    const Sp<R> zp3 = ldz(z + 3);
-    const Sp<R> hnext = (z + 1 < z1) ? ldh(z + 1) : Z;
+  load_fields_scaled(fld, own ? (size_t)z0 * nn + q : 0, c1, c2, shift, F);
+  double acc[2] = {0.0, 0.0};
+  __syncthreads();  // the zero fill precedes the first image store
+#pragma unroll
+  for (int s = 0; s < ZC; ++s) {
+    const int z = z0 + s;
+    if (z >= n) break;  // uniform over the workgroup
+    Sp<R>* tb = lds + (s & 1) * TS;
+    const bool nxt = (s + 1 < ZC) && (z + 1 < n);
+    // plane z + 3 is needed only while z + 3 <= z0 + ZC + 1
+    const Sp<R> zn = ld_sp(src + (size_t)(z + 3) * nn + q, own && z + 3 < n && s + 3 < ZC + 2);
+    const Sp<R> hn = ld_sp(src + (size_t)(z + 1) * nn + hq, nxt && hq >= 0);
     R Fn[NFLD];
-    if (own && z + 1 < z1) load_fields(fld, (size_t)(z + 1) * nn + q, Fn);
-    tile[2 * n + tid] = z00;
-    if (hslot >= 0) tile[hslot] = hcur;
+    load_fields_scaled(fld, (own && nxt) ? (size_t)(z + 1) * nn + q : 0, c1, c2, shift, Fn);
+    if (own) tb[slot] = z00;
+    if (hq >= 0) tb[hslot] = hc;
+    // one barrier per plane: the image written here was last read two
+    // planes ago, before every thread passed the previous plane's barrier
     __syncthreads();
     if (own) {
-      const int ct = 2 * n + tid;
-      const Sp<R> xm1 = x >= 1 ? tile[ct - 1] : Z;
-      const Sp<R> xm2 = x >= 2 ? tile[ct - 2] : Z;
-      const Sp<R> xp1 = x + 1 < n ? tile[ct + 1] : Z;
-      const Sp<R> xp2 = x + 2 < n ? tile[ct + 2] : Z;
-      // y neighbours need no test: rows outside the plane are the zero
-      // halo slots (hq = -1) or centre slots of threads without a point
-      const Sp<R> ym1 = tile[ct - n];
-      const Sp<R> ym2 = tile[ct - 2 * n];
-      const Sp<R> yp1 = tile[ct + n];
-      const Sp<R> yp2 = tile[ct + 2 * n];
-      const Sp<R> a0 = z00;
-      Sp<R> d[3];
-#define NHF_D(dd, P1, M1, P2, M2)                                  \
-  dd.ur = (R(8) * (P1.ur - M1.ur) - (P2.ur - M2.ur)) * c1;         \
-  dd.ui = (R(8) * (P1.ui - M1.ui) - (P2.ui - M2.ui)) * c1;         \
-  dd.dr = (R(8) * (P1.dr - M1.dr) - (P2.dr - M2.dr)) * c1;         \
-  dd.di = (R(8) * (P1.di - M1.di) - (P2.di - M2.di)) * c1;
-      NHF_D(d[0], xp1, xm1, xp2, xm2)
-      NHF_D(d[1], yp1, ym1, yp2, ym2)
-      NHF_D(d[2], zp1, zm1, zp2, zm2)
-#undef NHF_D
-      Sp<R> lap;
-      lap.ur = (R(16) * (xp1.ur + xm1.ur + yp1.ur + ym1.ur + zp1.ur + zm1.ur) -
-                (xp2.ur + xm2.ur + yp2.ur + ym2.ur + zp2.ur + zm2.ur) - R(90) * a0.ur) * c2;
-      lap.ui = (R(16) * (xp1.ui + xm1.ui + yp1.ui + ym1.ui + zp1.ui + zm1.ui) -
-                (xp2.ui + xm2.ui + yp2.ui + ym2.ui + zp2.ui + zm2.ui) - R(90) * a0.ui) * c2;
-      lap.dr = (R(16) * (xp1.dr + xm1.dr + yp1.dr + ym1.dr + zp1.dr + zm1.dr) -
-                (xp2.dr + xm2.dr + yp2.dr + ym2.dr + zp2.dr + zm2.dr) - R(90) * a0.dr) * c2;
-      lap.di = (R(16) * (xp1.di + xm1.di + yp1.di + ym1.di + zp1.di + zm1.di) -
-                (xp2.di + xm2.di + yp2.di + ym2.di + zp2.di + zm2.di) - R(90) * a0.di) * c2;
-      const size_t p = (size_t)z * nn + q;
-      const R f = F[0], gx = F[1], gy = F[2], gz = F[3], U = F[4] + shift, Wx = F[5], Wy = F[6], Wz = F[7];
-      const R t0r = -Wy * d[0].ur + Wz * d[0].di + Wx * d[1].ur - Wz * d[1].dr + Wy * d[2].dr - Wx * d[2].di;
-      const R t0i = -Wy * d[0].ui - Wz * d[0].dr + Wx * d[1].ui - Wz * d[1].di + Wy * d[2].di + Wx * d[2].dr;
-      const R t1r = -Wz * d[0].ui + Wy * d[0].dr - Wz * d[1].ur - Wx * d[1].dr + Wy * d[2].ur + Wx * d[2].ui;
-      const R t1i = Wz * d[0].ur + Wy * d[0].di - Wz * d[1].ui - Wx * d[1].di + Wy * d[2].ui - Wx * d[2].ur;
-      Sp<R> o;
-      o.ur = -f * lap.ur - (gx * d[0].ur + gy * d[1].ur + gz * d[2].ur) + U * a0.ur + t0i;
-      o.ui = -f * lap.ui - (gx * d[0].ui + gy * d[1].ui + gz * d[2].ui) + U * a0.ui - t0r;
-      o.dr = -f * lap.dr - (gx * d[0].dr + gy * d[1].dr + gz * d[2].dr) + U * a0.dr + t1i;
-      o.di = -f * lap.di - (gx * d[0].di + gy * d[1].di + gz * d[2].di) + U * a0.di - t1r;
-      dst[p] = o;
+      const Sp<R> o = hpsi_point<R>(zm2, zm1, z00, zp1, zp2, tb, slot, W, F);
+      nt_store<R>(dst + (size_t)z * nn + q, o);
       if (DOT) {
+        const Sp<R> a0 = z00;
         acc[0] += (double)a0.ur * o.ur + (double)a0.ui * o.ui + (double)a0.dr * o.dr + (double)a0.di * o.di;
         acc[1] += (double)a0.ur * o.ui - (double)a0.ui * o.ur + (double)a0.dr * o.di - (double)a0.di * o.dr;
       }
     }
-    __syncthreads();
     zm2 = zm1;
     zm1 = z00;
     z00 = zp1;
     zp1 = zp2;
-    zp2 = zp3;
-    hcur = hnext;
-    if (own && z + 1 < z1)
+    zp2 = zn;
+    hc = hn;
 #pragma unroll
-      for (int k = 0; k < NFLD; ++k) F[k] = Fn[k];
+    for (int k = 0; k < NFLD; ++k) F[k] = Fn[k];
   }
-  (void)N3;
-  if (DOT) block_atomic_add<2>(acc, dot + (size_t)dot_stride * c);
+  if (DOT) {
+    // workgroup sum of conj(in) . out: wavefront shuffles, then LDS over
+    // the blockDim.x / 64 wavefronts, one FP64 atomic per component
+    __shared__ double wsum[HPSI_MAXT / 64][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_down(acc[k], off, 64);
+    const int lane = tid & 63, wid = tid >> 6, nw = (int)(blockDim.x + 63) / 64;
+    if (lane == 0) {
+      wsum[wid][0] = acc[0];
+      wsum[wid][1] = acc[1];
+    }
+    __syncthreads();
+    if (tid < 2) {
+      double sum = 0.0;
+      for (int w = 0; w < nw; ++w) sum += wsum[w][tid];
+      atomicAdd(dot + (size_t)dot_stride * c + tid, sum);
+    }
+  }
 }
 
 template <typename R>
@@ -355,23 +415,42 @@ void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int nco
   size_t N3 = (size_t)n * n * n;
   R c1 = (R)(1.0 / (12.0 * h)), c2 = (R)(1.0 / (12.0 * h * h));
   hipEvent_t e0 = ev ? ev->start : nullptr, e1 = ev ? ev->stop : nullptr;
-  if (4 * n <= BS && hpsi_variant() == 1) {
-    // z chunks: enough workgroups to fill the device (about hpsi_wg_target()),
-    // but no chunk shorter than 4 planes (4 warm-up loads per chunk)
-    const int ntile = (n * n + BS - 1) / BS;
-    int nzc = (hpsi_wg_target() + ntile * ncols - 1) / (ntile * ncols);
-    nzc = std::max(1, std::min(nzc, n / 4));
-    const int zc = (n + nzc - 1) / nzc;
-    nzc = (n + zc - 1) / zc;
+  if (4 * n <= HPSI_MAXT && hpsi_variant() == 1) {
+    // rows per tile: as many whole rows as 256 threads hold, balanced over
+    // the tiles; z chunk: the longest of 14/10/8/6/4 planes that still gives
+    // >= 768 workgroups (three per CU), else 4
+    int rt = HPSI_MAXT / n;
+    const int ntile = (n + rt - 1) / rt;
+    rt = (n + ntile - 1) / ntile;  // >= 4 for n <= 64: the 4n halo loaders fit
+    static const int zcs[] = {14, 10, 8, 6, 4};
+    int zc = 4;
+    for (int zz : zcs)
+      if ((long)ntile * ((n + zz - 1) / zz) * ncols >= hpsi_wg_target()) {
+        zc = zz;
+        break;
+      }
+    const int nzc = (n + zc - 1) / zc;
     const int total = ntile * nzc * ncols, xcd = hpsi_xcd();
     dim3 g1((unsigned)(xcd ? (total + 7) / 8 * 8 : total));
-    size_t sh = sizeof(Sp<R>) * (BS + 4 * n);
-    if (dot)
-      LAUNCH_EV(e0, e1, (k_hpsi25<R, true>), g1, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
-                dot_stride, done, act, clist, ntile, nzc, ncols, xcd);
-    else
-      LAUNCH_EV(e0, e1, (k_hpsi25<R, false>), g1, dim3(BS), sh, st, in, out, ld, fld, n, c1, c2, (R)shift, zc, dot,
-                dot_stride, done, act, clist, ntile, nzc, ncols, xcd);
+    dim3 b1((unsigned)((rt * n + 63) / 64 * 64));
+    const size_t sh = sizeof(Sp<R>) * 2 * (rt + 4) * (n + 4);
+#define NHF_ROW(ZC)                                                                                               \
+  do {                                                                                                            \
+    if (dot)                                                                                                      \
+      LAUNCH_EV(e0, e1, (k_hpsi_row<R, true, ZC>), g1, b1, sh, st, in, out, ld, fld, n, rt, c1, c2, (R)shift, dot, \
+                dot_stride, done, act, clist, ntile, nzc, ncols, xcd);                                            \
+    else                                                                                                          \
+      LAUNCH_EV(e0, e1, (k_hpsi_row<R, false, ZC>), g1, b1, sh, st, in, out, ld, fld, n, rt, c1, c2, (R)shift,     \
+                dot, dot_stride, done, act, clist, ntile, nzc, ncols, xcd);                                       \
+  } while (0)
+    switch (zc) {
+      case 14: NHF_ROW(14); break;
+      case 10: NHF_ROW(10); break;
+      case 8: NHF_ROW(8); break;
+      case 6: NHF_ROW(6); break;
+      default: NHF_ROW(4); break;
+    }
+#undef NHF_ROW
     return;
   }
   dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
@@ -892,10 +971,11 @@ __global__ __launch_bounds__(GBS) void k_gram_lds(const Sp<float>* __restrict__ 
     act = true;
   } else {
     // lower tile triangle as row triples: (0,0) (1,0) (2,0) (3,0) (3,3) (4,0) (4,3) (5,0) (5,3)
-    const int rows[9] = {0, 1, 2, 3, 3, 4, 4, 5, 5}, cols[9] = {0, 0, 0, 0, 3, 0, 3, 0, 3};
+    // (wave 0..3: row wave, column 0; wave 4..8: row 3 + (wave - 3) / 2,
+    // column 3 for even waves, 0 for odd ones)
     act = wave < 9;
-    trow = act ? rows[wave] : 0;
-    tc0 = act ? cols[wave] : 0;
+    trow = !act ? 0 : wave < 4 ? wave : 3 + (wave - 3) / 2;
+    tc0 = (act && wave >= 4 && !(wave & 1)) ? 3 : 0;
   }
   act = act && I0 + 16 * trow < ka;
   bool on[3];
