@@ -128,6 +128,8 @@ typedef struct nhf_config {
   double mix_alpha;      /* Broyden linear part (0 -> 0.5) */
   int wstep;             /* NHF_W_JACOBI_CG (reference) | NHF_W_LEARNED */
   int wstep_iters;       /* learned W-block: minimal-residual steps, 1..8 (0 -> 1) */
+  int wstep_pcg;         /* learned W-block: Jacobi-PCG steps continuing from the net's
+                            correction, 0..1000 (0 = none) */
 } nhf_config;
 
 /* Density mixing.  The reference mixes linearly (iteration_data.cpp:928-995);
@@ -143,10 +145,12 @@ typedef struct nhf_config {
 /* The GCG W-block.  NHF_W_JACOBI_CG is the reference's Jacobi-PCG solve of
  * (H + sigma) w = (lambda + sigma) x (solver.cpp:727-745); NHF_W_LEARNED
  * runs wstep_iters preconditioned minimal-residual steps on the same system
- * with the context's learned 3D convolutional preconditioner (nhf_lcp_*).
- * When the mean residual reduction |r_k|/|r_0| of such a call exceeds 0.9
- * (a stall), that call falls back to the Jacobi-PCG W-block and so do the
- * next two calls of the species. */
+ * with the context's learned 3D convolutional preconditioner (nhf_lcp_*),
+ * then (wstep_pcg > 0) that many Jacobi-PCG steps of the reference's
+ * recurrence started from the net's correction instead of from x.
+ * When the mean residual reduction |r_k|/|r_0| of the net's steps exceeds
+ * 0.9 (a stall), that call falls back to the Jacobi-PCG W-block from x
+ * (gcg.steps steps) and so do the next two calls of the species. */
 #define NHF_W_JACOBI_CG 0
 #define NHF_W_LEARNED 1
 

@@ -59,7 +59,7 @@ class NhfConfig(ctypes.Structure):
         ("con", NhfConstraint * MAX_CON), ("constraints_tol", ctypes.c_double), ("precision", ctypes.c_int),
         ("stop_energy", ctypes.c_double), ("stop_window", ctypes.c_double),
         ("mixing", ctypes.c_int), ("mix_hist", ctypes.c_int), ("mix_alpha", ctypes.c_double),
-        ("wstep", ctypes.c_int), ("wstep_iters", ctypes.c_int),
+        ("wstep", ctypes.c_int), ("wstep_iters", ctypes.c_int), ("wstep_pcg", ctypes.c_int),
     ]
 
     @property
@@ -315,12 +315,34 @@ def ho_energies(nev, A):
 def workload(name, precision=PREC_FP64, seed=None):
     """The named synthetic workloads (nucleus, functional, lattice, states).
 
-    o16:  O-16 spherical HF, SLy4, box [-9,9] fm, 30 pts/axis, 12+12 states
-    ca40: Ca-40 HF+BCS, SLy4, box [-9,9] fm, 40 pts/axis, 32+32 states,
-          V0 = 200 + 100 U(0,1) MeV fm^3 from the seed
+    o16:   O-16 spherical HF, SLy4, box [-9,9] fm, 30 pts/axis, 12+12 states
+    ca40:  Ca-40 HF+BCS, SLy4, box [-9,9] fm, 40 pts/axis, 32+32 states,
+           V0 = 200 + 100 U(0,1) MeV fm^3 from the seed
+    sn120: Sn-120 HF+BCS, SLy4, box [-9.6,9.6] fm, 48 pts/axis, 70 neutron
+           + 56 proton states, V0 as ca40.  With 70 states for 70 neutrons
+           there is no empty neutron level, so (as the reference's input
+           check requires HFbasisSize > particles for a paired species)
+           only the protons are paired
+    pb208: Pb-208 HFB (two-basis, volume delta pairing, V0 = 500 MeV fm^3 as
+           in the reference's HFB input), SLy4, box [-11.2,11.2] fm,
+           56 pts/axis, 160 neutron + 110 proton states, startHFBIter 3
+    The deformation-curve batch (Mg-24 and Si-28) is curve_batch().
     """
     import random
 
+    if name == "sn120":
+        rng = random.Random(1234 if seed is None else seed)
+        V0 = 200.0 + 100.0 * rng.random()
+        d = {"functional": "SLy4", "maxIterations": 600, "energyTol": 1e-6,
+             "box": {"boxSize": 19.2, "axisMeshPoints": 48}, "nucleus": {"A": 120, "Z": 50},
+             "pairing": {"type": "BCS", "proton": {"V0": V0, "HFbasisSize": 56, "window": 5.0}}}
+        return config_from_input(d, precision)
+    if name == "pb208":
+        d = {"functional": "SLy4", "maxIterations": 600, "energyTol": 1e-6,
+             "box": {"boxSize": 22.4, "axisMeshPoints": 56}, "nucleus": {"A": 208, "Z": 82},
+             "pairing": {"type": "HFB", "neutron": {"HFbasisSize": 160, "V0": 500.0},
+                         "proton": {"HFbasisSize": 110, "V0": 500.0}}}
+        return config_from_input(d, precision)
     if name == "o16":
         d = {"functional": "SLy4", "maxIterations": 600, "energyTol": 1e-6,
              "box": {"boxSize": 18.0, "axisMeshPoints": 30}, "nucleus": {"A": 16, "Z": 8}}
@@ -335,3 +357,24 @@ def workload(name, precision=PREC_FP64, seed=None):
              "pairing": {"type": "BCS", "window": 5.0, "V0": V0, "HFbasisSize": 32}}
         return config_from_input(d, precision)
     raise KeyError(name)
+
+
+CURVE_BETAS = tuple(-0.4 + 1.2 * i / 7 for i in range(8))
+
+
+def curve_batch(precision=PREC_FP64, betas=CURVE_BETAS, c_q20=0.1):
+    """The batched quadrupole-constrained curve: Mg-24 on a 32^3 lattice and
+    Si-28 on a 40^3 lattice (box [-9,9] fm), SkM*, 20 states per species, no
+    pairing, at the beta2 values `betas` (eight evenly spaced over
+    [-0.4, 0.8]) with the curve's constraint list (main.cpp:175-182) and a
+    Q20 stiffness of c_q20 MeV.  Returns [(label, beta2, cfg), ...]."""
+    out = []
+    for A, Z, n in ((24, 12, 32), (28, 14, 40)):
+        d = {"functional": "SKMstar", "maxIterations": 600, "energyTol": 1e-6,
+             "box": {"boxSize": 18.0, "axisMeshPoints": n}, "nucleus": {"A": A, "Z": Z}}
+        for b in betas:
+            cfg = config_from_input(d, precision)
+            cfg.M_n = cfg.M_p = 20
+            set_curve_constraints(cfg, mu20_from_beta(b, A), c_q20=c_q20)
+            out.append((f"{'Mg' if Z == 12 else 'Si'}-{A} {n}^3 beta2={b:+.4f}", b, cfg))
+    return out

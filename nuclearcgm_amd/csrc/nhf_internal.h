@@ -51,8 +51,11 @@ void launch_hpsi(hipStream_t st, const Sp<R>* in, Sp<R>* out, size_t ld, int nco
 
 // block Jacobi-PCG kernels (Eigen ConjugateGradient restated per column)
 template <typename R>
+// from_w = 0: start from w0 = X (W <- X, T = (H + shift) X); from_w = 1: W
+// already holds the start vector and T = (H + shift) W
 void launch_cg_init(hipStream_t st, const Sp<R>* X, const double* lam_shift, Sp<R>* W, Sp<R>* Rr, Sp<R>* P,
-                    const Sp<R>* T, const R* invd, size_t N3, size_t ld, int ncols, double* s0, double* rhs2);
+                    const Sp<R>* T, const R* invd, size_t N3, size_t ld, int ncols, double* s0, double* rhs2,
+                    int from_w = 0);
 // clist (optional): grid.y enumerates clist[0..ncols) instead of 0..ncols
 template <typename R>
 void launch_cg_update(hipStream_t st, Sp<R>* W, Sp<R>* Rr, const Sp<R>* P, const Sp<R>* T, const R* invd, size_t N3,

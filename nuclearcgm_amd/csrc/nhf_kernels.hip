@@ -477,7 +477,8 @@ template <typename R>
 __global__ __launch_bounds__(BS) void k_cg_init(const Sp<R>* __restrict__ X, const double* __restrict__ lam_shift,
                                                 Sp<R>* __restrict__ W, Sp<R>* __restrict__ Rr, Sp<R>* __restrict__ P,
                                                 const Sp<R>* __restrict__ T, const R* __restrict__ invd, size_t N3,
-                                                size_t ld, double* __restrict__ s0, double* __restrict__ rhs2) {
+                                                size_t ld, double* __restrict__ s0, double* __restrict__ rhs2,
+                                                int from_w) {
   const int c = blockIdx.y;
   const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
   double acc[3] = {0.0, 0.0, 0.0};  // |b|^2, |r|^2, r^H z
@@ -485,7 +486,7 @@ __global__ __launch_bounds__(BS) void k_cg_init(const Sp<R>* __restrict__ X, con
     const size_t o = (size_t)c * ld + p;
     const Sp<R> x = X[o], t = T[o];
     const R ls = (R)lam_shift[c];
-    W[o] = x;
+    if (!from_w) W[o] = x;  // else W holds the start vector and T = (H + shift) W
     Sp<R> b{x.ur * ls, x.ui * ls, x.dr * ls, x.di * ls};
     Sp<R> r{b.ur - t.ur, b.ui - t.ui, b.dr - t.dr, b.di - t.di};
     Rr[o] = r;
@@ -507,9 +508,11 @@ __global__ __launch_bounds__(BS) void k_cg_init(const Sp<R>* __restrict__ X, con
 
 template <typename R>
 void launch_cg_init(hipStream_t st, const Sp<R>* X, const double* lam_shift, Sp<R>* W, Sp<R>* Rr, Sp<R>* P,
-                    const Sp<R>* T, const R* invd, size_t N3, size_t ld, int ncols, double* s0, double* rhs2) {
+                    const Sp<R>* T, const R* invd, size_t N3, size_t ld, int ncols, double* s0, double* rhs2,
+                    int from_w) {
   dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
-  hipLaunchKernelGGL((k_cg_init<R>), grid, dim3(BS), 0, st, X, lam_shift, W, Rr, P, T, invd, N3, ld, s0, rhs2);
+  hipLaunchKernelGGL((k_cg_init<R>), grid, dim3(BS), 0, st, X, lam_shift, W, Rr, P, T, invd, N3, ld, s0, rhs2,
+                     from_w);
 }
 
 __global__ void k_cg_check0(const double* __restrict__ s0, const double* __restrict__ rhs2, double tol, int ncols,
@@ -607,7 +610,7 @@ void launch_cg_dir(hipStream_t st, const Sp<R>* Rr, Sp<R>* P, const R* invd, siz
 
 #define INST_CG(R)                                                                                                    \
   template void launch_cg_init<R>(hipStream_t, const Sp<R>*, const double*, Sp<R>*, Sp<R>*, Sp<R>*, const Sp<R>*,    \
-                                  const R*, size_t, size_t, int, double*, double*);                                  \
+                                  const R*, size_t, size_t, int, double*, double*, int);                             \
   template void launch_cg_update<R>(hipStream_t, Sp<R>*, Sp<R>*, const Sp<R>*, const Sp<R>*, const R*, size_t,       \
                                     size_t, int, const double*, double*, const int*, const int*);                    \
   template void launch_cg_dir<R>(hipStream_t, const Sp<R>*, Sp<R>*, const R*, size_t, size_t, int, const double*,    \

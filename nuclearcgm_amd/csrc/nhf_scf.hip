@@ -242,7 +242,7 @@ static int setup_problem(Problem& P) {
   P.M[1] = c.M_p;
   P.Mmax = std::max(c.M_n, c.M_p);
   P.kmax = 3 * P.Mmax;
-  P.steps_cap = std::max(1, c.gcg.steps);
+  P.steps_cap = std::max(1, std::max(c.gcg.steps, c.wstep_pcg));
   P.prec = c.precision;
   P.massCorr = c.com_corr ? MNUC * (double)c.A / (double)(c.A - 1) : MNUC;
   const size_t N3 = P.N3;
@@ -549,17 +549,20 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
         }
       }
     }
-    if (!w_done) {
+    // the Jacobi-PCG W-block: from w0 = x (the reference), or from the
+    // learned correction when wstep_pcg > 0 (`refine`)
+    const bool refine = w_done && P.cfg.wstep_pcg > 0;
+    if (!w_done || refine) {
     launch_invdiag<R>(st, fld, N3, P.h, shift, invd);
-    CHK(hpsi_t<R>(P, Wk, q, X, T, M, shift));
+    CHK(hpsi_t<R>(P, Wk, q, refine ? Wb : X, T, M, shift));
     std::vector<double> ls(M);
     for (int c = 0; c < M; ++c) ls[c] = Lam[c] + shift;
     HIPCHK(hipMemcpyAsync(Wk.lam_shift, ls.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
-    const int steps = std::min(prm.steps, P.steps_cap);
+    const int steps = std::min(refine ? P.cfg.wstep_pcg : prm.steps, P.steps_cap);
     HIPCHK(hipMemsetAsync(Wk.cg_scal, 0, sizeof(double) * (size_t)(steps + 1) * M * CG_NS, st));
     HIPCHK(hipMemsetAsync(Wk.rhs2, 0, sizeof(double) * M, st));
     HIPCHK(hipMemsetAsync(Wk.its, 0, sizeof(int) * M, st));
-    launch_cg_init<R>(st, X, Wk.lam_shift, Wb, Rr, Pd, T, invd, N3, ld, M, Wk.cg_scal, Wk.rhs2);
+    launch_cg_init<R>(st, X, Wk.lam_shift, Wb, Rr, Pd, T, invd, N3, ld, M, Wk.cg_scal, Wk.rhs2, refine ? 1 : 0);
     launch_cg_check0(st, Wk.cg_scal, Wk.rhs2, prm.cg_tol, M, Wk.thr, Wk.done);
     // active-column list: the CG kernels enumerate only the columns that
     // were not yet converged at the last host check (every 4 iterations)
@@ -599,7 +602,8 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
       for (int c = 0; c < M; ++c) info->cg_iters += its[c];
     }
     // training pairs (residual, converged correction) for the learned net
-    if (P.lcp && P.lcp->collect_left > 0 && (Wk.gcg_calls - 1) % std::max(1, P.lcp->collect_every) == 0) {
+    if (!refine && P.lcp && P.lcp->collect_left > 0 &&
+        (Wk.gcg_calls - 1) % std::max(1, P.lcp->collect_every) == 0) {
       launch_lcp_planes<R>(st, fld, N3, P.h, shift, Wk.kap, Wk.hf);
       CHK(hpsi_t<R>(P, Wk, q, X, T, M, shift));
       CHK(lcp_add_pairs<R>(*P.lcp, st, T, X, Wb, Wk.lam_shift, M, ld, Wk.kap, Wk.hf, P.n, Wk.pscr));
@@ -1368,6 +1372,16 @@ static int species_step(Problem& P, int q, const FieldParams& F, hipEvent_t dens
 }
 
 // The self-consistent loop (main.cpp:199-342).
+// optional per-iteration progress log: the file named by the environment
+// variable NHF_SCF_LOG (appended to; one line per HF iteration)
+static FILE* scf_log() {
+  static FILE* f = [] {
+    const char* e = std::getenv("NHF_SCF_LOG");
+    return e && *e ? std::fopen(e, "a") : nullptr;
+  }();
+  return f;
+}
+
 template <typename R>
 static int scf_dev(Problem& P, std::vector<double> eps[2], nhf_result* res, double* trace, int cap) {
   hipStream_t st = P.st;
@@ -1442,6 +1456,11 @@ static int scf_dev(Problem& P, std::vector<double> eps[2], nhf_result* res, doub
       trace[4 * it + 1] = Ehf;
       trace[4 * it + 2] = mx;
       trace[4 * it + 3] = el;
+    }
+    if (FILE* lg = scf_log()) {
+      std::fprintf(lg, "A=%d Z=%d n=%d prec=%d it=%d E_int=%.9f dE=%.3e dEps=%.3e t=%.3f\n", c.A, c.Z, c.n,
+                   c.precision, it, Eint, Eint - Eint_prev, mx, el);
+      std::fflush(lg);
     }
     if (tstop < 0 && std::isfinite(c.stop_energy) && std::abs(Eint - c.stop_energy) <= c.stop_window) tstop = el;
     if (std::abs(Eint - Eint_prev) < c.energy_tol && std::abs(Ehf - Ehf_prev) < c.energy_tol && cc) {
@@ -1577,7 +1596,7 @@ int nhf_problem(void* ctx, const nhf_config* cfg, int* pid) {
   }
   if ((cfg->mixing != NHF_MIX_LINEAR && cfg->mixing != NHF_MIX_BROYDEN) || cfg->mix_hist < 0 ||
       cfg->mix_hist > NHF_MAX_HIST || cfg->mix_alpha < 0 || (cfg->wstep != NHF_W_JACOBI_CG && cfg->wstep != NHF_W_LEARNED) ||
-      cfg->wstep_iters < 0 || cfg->wstep_iters > LW_MAX_ITERS)
+      cfg->wstep_iters < 0 || cfg->wstep_iters > LW_MAX_ITERS || cfg->wstep_pcg < 0 || cfg->wstep_pcg > 1000)
     return NHF_EINVAL;
   Ctx* c = (Ctx*)ctx;
   HIPCHK(hipSetDevice(c->device));

@@ -16,6 +16,18 @@ hardware counters):
     python -m nuclearcgm_amd.measure gram [--reps R]
         R block products of device-resident FP32 state blocks of the Ca-40
         lattice for the shapes of the GCG pass (k x k Hermitian, k x k, ...)
+    python -m nuclearcgm_amd.measure configs [--which o16,curve,sn120,pb208]
+        every workload run twice: the reference iteration in FP64 (linear
+        mixing, 80-step Jacobi-PCG W-block) to convergence, which fixes
+        E_ref, and the accelerated path of the benchmark (mixed precision,
+        Broyden mixing, learned W-block + Jacobi-PCG refinement; the net
+        trained once, as the benchmark does); one JSON line per workload
+        with the time to the 1 keV window of E_ref and the errors of E_tot,
+        the occupied single-particle energies, the rms radii and beta2.
+        curve: the 16 constrained Mg-24 (32^3) / Si-28 (40^3) points in one
+        concurrent batch per path, each from the Woods-Saxon start at its
+        own beta2.  pb208: the accelerated path as a batch of three seeded
+        guess perturbations.
 """
 import argparse
 import os
@@ -26,7 +38,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["solve", "copy", "stencil", "gram"])
+    ap.add_argument("what", choices=["solve", "copy", "stencil", "gram", "configs"])
+    ap.add_argument("--which", default="o16,curve,sn120,pb208")
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--wstep-iters", type=int, default=1)
     ap.add_argument("--no-train", action="store_true")
@@ -49,6 +62,9 @@ def main(argv=None):
     from nuclearcgm_amd import hip
 
     ctx = hip.Context(0)
+    if a.what == "configs":
+        measure_configs(ctx, a.which.split(","))
+        return
     if a.what == "stencil":
         cfg = C.workload("ca40", precision=C.PREC_MIXED)
         p = ctx.problem(cfg)
@@ -94,6 +110,96 @@ def main(argv=None):
         print(f"live {k}: {v['launches']} launches, {v['ms'] * 1e3 / n:.2f} us avg, "
               f"{v['bytes'] / n / 1e6:.2f} MB algorithmic per launch, "
               f"{v['bytes'] / max(v['ms'], 1e-12) / 1e6:.1f} GB/s", flush=True)
+
+
+def _errors(bench, C, r, ref, cfg):
+    """Errors of result r against the FP64 result ref (both scf_run tuples)."""
+    sm, so = r[4].summary(), ref[4].summary()
+    N, Z = cfg.A - cfg.Z, cfg.Z
+    return {"dE_keV": (sm["E_int"] - so["E_int"]) * 1e3,
+            "max_deps_occupied_MeV": max(bench.spe_errors(r[2], ref[2], N)[0], bench.spe_errors(r[3], ref[3], Z)[0]),
+            "d_rms_n_fm": sm["rms_n"] - so["rms_n"], "d_rms_p_fm": sm["rms_p"] - so["rms_p"],
+            "d_beta2": sm["beta2"] - so["beta2"]}
+
+
+def measure_configs(ctx, which):
+    import json
+    import time
+
+    import numpy as np
+
+    import bench
+    from nuclearcgm_amd import config as C
+
+    setup = bench.train_preconditioner(ctx)
+    print(json.dumps({"setup": setup}), flush=True)
+
+    def accel(c):
+        c.mixing = C.MIX_BROYDEN
+        c.wstep, c.wstep_iters, c.wstep_pcg = C.W_LEARNED, 1, bench.WSTEP_PCG
+        return c
+
+    for w in which:
+        if w in ("o16", "sn120", "pb208"):
+            c64 = C.workload(w, precision=C.PREC_FP64)
+            c64.max_iter = 1500
+            g = bench.make_guess(c64)
+            t = time.time()
+            ref = ctx.problem(c64).scf_run(*g)
+            w64 = time.time() - t
+            e_ref = ref[4].terms[C.E_INT]
+            t64, k64 = bench.time_to_window(ref[5], e_ref)
+            cm = accel(C.workload(w, precision=C.PREC_MIXED))
+            cm.max_iter = 600
+            if w == "pb208":
+                # three seeded guess perturbations of amplitude 1e-3, concurrently
+                runs = [(ctx.problem(cm), bench.make_guess(cm, seed=s)) for s in (7, 8, 9)]
+                t = time.time()
+                outs = ctx.batch_run(runs)
+                wall = time.time() - t
+                tk = [bench.time_to_window(o[5], e_ref) for o in outs]
+                line = {"workload": w, "fp64": {"seconds_to_1keV": t64, "iterations_to_1keV": k64,
+                                                 "iterations": ref[4].iterations, "wall_s": w64, "E_int": e_ref},
+                        "accelerated_batch_of_3": {"wall_s": wall, "seconds_to_1keV": [x[0] for x in tk],
+                                                   "iterations_to_1keV": [x[1] for x in tk],
+                                                   "errors": [_errors(bench, C, o, ref, c64) for o in outs]}}
+            else:
+                p = ctx.problem(cm)
+                p.scf_run(*g)  # warm-up
+                t = time.time()
+                r = p.scf_run(*g)
+                wall = time.time() - t
+                tm, km = bench.time_to_window(r[5], e_ref)
+                line = {"workload": w, "fp64": {"seconds_to_1keV": t64, "iterations_to_1keV": k64,
+                                                 "iterations": ref[4].iterations, "wall_s": w64, "E_int": e_ref},
+                        "accelerated": {"seconds_to_1keV": tm, "iterations_to_1keV": km, "wall_s": wall,
+                                        "speedup": t64 / tm if tm > 0 else None, **_errors(bench, C, r, ref, c64)}}
+            print(json.dumps(line), flush=True)
+        elif w == "curve":
+            res = {}
+            for prec in (C.PREC_FP64, C.PREC_MIXED):
+                items = C.curve_batch(prec)
+                runs = []
+                for label, beta, c in items:
+                    c.max_iter = 600
+                    if prec == C.PREC_MIXED:
+                        accel(c)
+                    pr = ctx.problem(c)
+                    runs.append((pr, pr.ws_guess(beta2=beta)))
+                t = time.time()
+                res[prec] = (ctx.batch_run(runs), time.time() - t, items)
+            (o64, w64, items), (om, wm, _) = res[C.PREC_FP64], res[C.PREC_MIXED]
+            pts = []
+            for (label, beta, c), r64, rm in zip(items, o64, om):
+                e_ref = r64[4].terms[C.E_INT]
+                pts.append({"point": label, "fp64_converged": bool(r64[4].converged),
+                            "fp64_iterations": r64[4].iterations, "E_int": e_ref, "beta2": r64[4].beta2,
+                            "accel_converged": bool(rm[4].converged), "accel_iterations": rm[4].iterations,
+                            "fp64_seconds_to_1keV": bench.time_to_window(r64[5], e_ref)[0],
+                            "accel_seconds_to_1keV": bench.time_to_window(rm[5], e_ref)[0],
+                            **_errors(bench, C, rm, r64, c)})
+            print(json.dumps({"workload": "curve", "fp64_batch_wall_s": w64, "accel_batch_wall_s": wm,
+                              "points": pts}), flush=True)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,78 @@
+"""Runs the FP64 CPU oracle on the larger workloads from the seeded HO guess
+and writes JSON fixtures of the results (energies, single-particle
+energies, observables, the E_int trace):
+
+    python tests/golden/make_goldens.py curve   # two points of the Mg/Si curve batch
+    python tests/golden/make_goldens.py sn120   # Sn-120 48^3 HF+BCS to convergence
+    python tests/golden/make_goldens.py pb208   # Pb-208 56^3 HFB, first iterations
+
+curve: Mg-24 (32^3) at beta2 = +0.4571 and Si-28 (40^3) at beta2 = -0.2286,
+SkM*, 20+20 states, the curve's constraint list with a Q20 stiffness of
+0.1 MeV, each run to convergence (|dE| < 1e-6 MeV and constraints within
+1e-3).  sn120: to convergence.  pb208: the first PB208_ITERS HF iterations
+(the BCS seed steps and the first HFB steps), whose E_int trace pins the
+densities, fields, pairing and energies of the full-size lattice.
+
+Takes tens of minutes per workload on 8 cores (OMP_NUM_THREADS)."""
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+import _oracle as O  # noqa: E402
+from nuclearcgm_amd import config as C  # noqa: E402
+
+import bench  # noqa: E402
+
+CURVE_POINTS = (5, 9)  # indices into config.curve_batch(): Mg-24 beta2=+0.4571, Si-28 beta2=-0.2286
+PB208_ITERS = 5
+
+
+def run(cfg, label, extra=None):
+    guess = bench.make_guess(cfg)
+    t = time.time()
+    pn, pp, en, ep, res, trace = O.scf_run(cfg, *guess)
+    s = res.summary()
+    s.update({"label": label, "eps_n": list(map(float, en)), "eps_p": list(map(float, ep)),
+              "wall_s": time.time() - t, "trace_E_int": [float(x) for x in trace[:, 0]],
+              "trace_E_HF": [float(x) for x in trace[:, 1]], "seed_guess": 7,
+              "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))})
+    if extra:
+        s.update(extra)
+    print(label, s["E_int"], s["iterations"], s["converged"], f"{s['wall_s']:.0f} s", flush=True)
+    return s
+
+
+def main(what):
+    if what == "curve":
+        batch = C.curve_batch()
+        out = []
+        for i in CURVE_POINTS:
+            label, beta, cfg = batch[i]
+            cfg.max_iter = 600
+            out.append(run(cfg, label, {"index": i, "beta2_target": beta}))
+        path = os.path.join(HERE, "curve_oracle.json")
+        data = out
+    elif what == "sn120":
+        cfg = C.workload("sn120")
+        cfg.max_iter = 600
+        data = run(cfg, "Sn-120 48^3 HF+BCS", {"V0_p": cfg.pair_p.V0})
+        path = os.path.join(HERE, "sn120_oracle.json")
+    elif what == "pb208":
+        cfg = C.workload("pb208")
+        cfg.max_iter = PB208_ITERS
+        data = run(cfg, f"Pb-208 56^3 HFB, first {PB208_ITERS} iterations")
+        path = os.path.join(HERE, "pb208_oracle_start.json")
+    else:
+        raise SystemExit(__doc__)
+    with open(path, "w") as fh:
+        json.dump(data, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "")

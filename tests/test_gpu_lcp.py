@@ -82,11 +82,11 @@ def test_lcp_training_and_ablation(ctx):
     g = bench.make_guess(cfg)
     gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ca40_oracle.json")))["E_int"]
 
-    def run(wstep, steps, iters=1):
+    def run(wstep, steps, iters=1, pcg=0):
         c = C.workload("ca40", precision=C.PREC_MIXED)
         c.max_iter = 300
         c.mixing = C.MIX_BROYDEN
-        c.wstep, c.wstep_iters, c.gcg.steps = wstep, iters, steps
+        c.wstep, c.wstep_iters, c.gcg.steps, c.wstep_pcg = wstep, iters, steps, pcg
         p = ctx.problem(c)
         res = p.scf_run(*g)
         _, k = bench.time_to_window(res[5], gold)
@@ -119,3 +119,13 @@ def test_lcp_training_and_ablation(ctx):
         assert s_u[q]["mean_ratio"] > 0.9
     # and the SCF needs far fewer iterations than with one Jacobi step
     assert 0 < k_l < 0.5 * k_j
+    # the net's correction as the start of a short Jacobi-PCG (4 steps)
+    # beats the same 4 steps started from x (16 vs 21 HF iterations to
+    # 1 keV measured) and reaches the same fixed point
+    r_h, k_h, s_h = run(C.W_LEARNED, 80, pcg=4)
+    r_4, k_4, _ = run(C.W_JACOBI_CG, 4)
+    print(f"iterations to 1 keV: learned + 4 PCG steps {k_h} (fallbacks {s_h['n']['fallbacks']}), "
+          f"4 PCG steps {k_4}")
+    for r in (r_h, r_4):
+        assert abs(r.terms[C.E_INT] - gold) < 1e-3
+    assert 0 < k_h < k_4
