@@ -8,13 +8,63 @@
 // tridiagonal eigenproblem is solved by the implicit QL method with Wilkinson
 // shifts, and the eigenvectors are transformed back.  Eigenvalues ascending.
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
+#include <thread>
 
 #include "nhf_internal.h"
 
 namespace nhf {
+
+// A team of host threads for the O(k^3) loops of large bases (k >= 192, e.g.
+// the 480-column basis of Pb-208): the caller is member 0, the others are
+// started by run() and joined at its end; barrier() is a sense-reversing
+// spin barrier between the phases of one Householder step.  With one member
+// everything runs inline.
+class Team {
+ public:
+  explicit Team(int n) : n_(std::max(1, n)) {}
+  int size() const { return n_; }
+  template <class F>
+  void run(F&& f) {
+    std::vector<std::thread> th;
+    th.reserve(n_ - 1);
+    for (int t = 1; t < n_; ++t) th.emplace_back([&f, t] { f(t); });
+    f(0);
+    for (auto& x : th) x.join();
+  }
+  void barrier() {
+    if (n_ == 1) return;
+    const int g = gen_.load(std::memory_order_acquire);
+    if (cnt_.fetch_add(1, std::memory_order_acq_rel) == n_ - 1) {
+      cnt_.store(0, std::memory_order_relaxed);
+      gen_.store(g + 1, std::memory_order_release);
+    } else {
+      while (gen_.load(std::memory_order_acquire) == g) __builtin_ia32_pause();
+    }
+  }
+
+ private:
+  int n_;
+  std::atomic<int> cnt_{0}, gen_{0};
+};
+
+// members for a k x k problem: one below k = 192, else up to 8 (half the
+// host's hardware threads: the two species solve concurrently); the
+// environment variable NHF_HOST_THREADS sets the count for large problems
+static int team_size(int k) {
+  if (k < 192) return 1;
+  static const int n = [] {
+    const char* e = std::getenv("NHF_HOST_THREADS");
+    if (e && *e) return std::max(1, std::min(64, std::atoi(e)));
+    const int hw = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(8, hw / 2));
+  }();
+  return n;
+}
 
 // sqrt(a^2 + b^2) without the over/underflow guards of hypot (a library
 // call that dominated the QL sweep): the projected matrices have entries of
@@ -175,77 +225,114 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V, int nvec) {
   std::vector<cd> tau(k, 0.0);
   std::vector<double> vsr((size_t)k * k, 0.0), vsi((size_t)k * k, 0.0);  // reflector i in column i
   std::vector<double> xr(k), xi(k);
-  for (int i = 0; i < k - 1; ++i) {
-    const int m = k - i - 1, o = i + 1;
-    double* vr = &vsr[(size_t)i * k];
-    double* vi = &vsi[(size_t)i * k];
-    cd alpha(Ar[(size_t)i * k + o], Ai[(size_t)i * k + o]);
-    double xn2 = 0.0;
-    for (int r = o + 1; r < k; ++r) xn2 += Ar[(size_t)i * k + r] * Ar[(size_t)i * k + r] + Ai[(size_t)i * k + r] * Ai[(size_t)i * k + r];
-    cd ti = 0.0;
-    vr[0] = 1.0;
-    vi[0] = 0.0;
-    if (!(xn2 == 0.0 && alpha.imag() == 0.0)) {
-      const double beta = -std::copysign(std::sqrt(std::norm(alpha) + xn2), alpha.real());
-      ti = cd((beta - alpha.real()) / beta, -alpha.imag() / beta);
-      const cd sc = 1.0 / (alpha - beta);
-      for (int r = o + 1; r < k; ++r) {
-        const cd a(Ar[(size_t)i * k + r], Ai[(size_t)i * k + r]);
-        const cd v = a * sc;
-        vr[r - o] = v.real();
-        vi[r - o] = v.imag();
-      }
-      alpha = beta;
-    }
-    e[i] = alpha.real();
-    if (ti != cd(0.0, 0.0)) {
-      // y = A22 v from the lower triangle of A22 (the upper one is its
-      // conjugate), x = tau y
-      std::fill(xr.begin(), xr.begin() + m, 0.0);
-      std::fill(xi.begin(), xi.begin() + m, 0.0);
-      for (int c = 0; c < m; ++c) {
-        const double* ar = &Ar[(size_t)(o + c) * k + o];
-        const double* ai = &Ai[(size_t)(o + c) * k + o];
-        const double br = vr[c], bi = vi[c];
-        double sr = ar[c] * br, si = ar[c] * bi;  // real diagonal
-        for (int r = c + 1; r < m; ++r) {
-          xr[r] += ar[r] * br - ai[r] * bi;
-          xi[r] += ar[r] * bi + ai[r] * br;
-          // conj(A[r, c]) v[r]
-          sr += ar[r] * vr[r] + ai[r] * vi[r];
-          si += ar[r] * vi[r] - ai[r] * vr[r];
+  Team team(team_size(k));
+  const int T = team.size();
+  // per-member partial sums of y = A22 v (member 0 accumulates into xr, xi)
+  std::vector<double> pr((size_t)T * k), pi((size_t)T * k);
+  cd ti_sh = 0.0;  // the current step's tau, published by member 0
+  team.run([&](int tid) {
+    for (int i = 0; i < k - 1; ++i) {
+      const int m = k - i - 1, o = i + 1;
+      double* vr = &vsr[(size_t)i * k];
+      double* vi = &vsi[(size_t)i * k];
+      if (tid == 0) {
+        cd alpha(Ar[(size_t)i * k + o], Ai[(size_t)i * k + o]);
+        double xn2 = 0.0;
+        for (int r = o + 1; r < k; ++r)
+          xn2 += Ar[(size_t)i * k + r] * Ar[(size_t)i * k + r] + Ai[(size_t)i * k + r] * Ai[(size_t)i * k + r];
+        cd ti = 0.0;
+        vr[0] = 1.0;
+        vi[0] = 0.0;
+        if (!(xn2 == 0.0 && alpha.imag() == 0.0)) {
+          const double beta = -std::copysign(std::sqrt(std::norm(alpha) + xn2), alpha.real());
+          ti = cd((beta - alpha.real()) / beta, -alpha.imag() / beta);
+          const cd sc = 1.0 / (alpha - beta);
+          for (int r = o + 1; r < k; ++r) {
+            const cd a(Ar[(size_t)i * k + r], Ai[(size_t)i * k + r]);
+            const cd v = a * sc;
+            vr[r - o] = v.real();
+            vi[r - o] = v.imag();
+          }
+          alpha = beta;
         }
-        xr[c] += sr;
-        xi[c] += si;
+        e[i] = alpha.real();
+        ti_sh = ti;
       }
-      double xvr = 0.0, xvi = 0.0;  // (tau y)^H v
-      for (int r = 0; r < m; ++r) {
-        const double tr = ti.real() * xr[r] - ti.imag() * xi[r], tim = ti.real() * xi[r] + ti.imag() * xr[r];
-        xr[r] = tr;
-        xi[r] = tim;
-        xvr += tr * vr[r] + tim * vi[r];
-        xvi += tr * vi[r] - tim * vr[r];
-      }
-      const cd al2 = -0.5 * ti * cd(xvr, xvi);
-      for (int r = 0; r < m; ++r) {
-        xr[r] += al2.real() * vr[r] - al2.imag() * vi[r];
-        xi[r] += al2.real() * vi[r] + al2.imag() * vr[r];
-      }
-      // A22 -= v x^H + x v^H, lower triangle
-      for (int c = 0; c < m; ++c) {
-        double* ar = &Ar[(size_t)(o + c) * k + o];
-        double* ai = &Ai[(size_t)(o + c) * k + o];
-        const double xcr = xr[c], xci = -xi[c], vcr = vr[c], vci = -vi[c];  // conjugates
-        for (int r = c; r < m; ++r) {
-          ar[r] -= vr[r] * xcr - vi[r] * xci + xr[r] * vcr - xi[r] * vci;
-          ai[r] -= vr[r] * xci + vi[r] * xcr + xr[r] * vci + xi[r] * vcr;
+      team.barrier();
+      const cd ti = ti_sh;
+      if (ti != cd(0.0, 0.0)) {
+        // y = A22 v from the lower triangle of A22 (the upper one is its
+        // conjugate): columns c = tid, tid + T, ... into this member's
+        // partial sums, then the partial sums added row block by row block
+        double* yr = T == 1 ? xr.data() : &pr[(size_t)tid * k];
+        double* yi = T == 1 ? xi.data() : &pi[(size_t)tid * k];
+        std::fill(yr, yr + m, 0.0);
+        std::fill(yi, yi + m, 0.0);
+        for (int c = tid; c < m; c += T) {
+          const double* ar = &Ar[(size_t)(o + c) * k + o];
+          const double* ai = &Ai[(size_t)(o + c) * k + o];
+          const double br = vr[c], bi = vi[c];
+          double sr = ar[c] * br, si = ar[c] * bi;  // real diagonal
+          for (int r = c + 1; r < m; ++r) {
+            yr[r] += ar[r] * br - ai[r] * bi;
+            yi[r] += ar[r] * bi + ai[r] * br;
+            // conj(A[r, c]) v[r]
+            sr += ar[r] * vr[r] + ai[r] * vi[r];
+            si += ar[r] * vi[r] - ai[r] * vr[r];
+          }
+          yr[c] += sr;
+          yi[c] += si;
         }
+        if (T > 1) {
+          team.barrier();
+          const int r0 = (int)((long)m * tid / T), r1 = (int)((long)m * (tid + 1) / T);
+          for (int r = r0; r < r1; ++r) {
+            double sr = 0.0, si = 0.0;
+            for (int t = 0; t < T; ++t) {
+              sr += pr[(size_t)t * k + r];
+              si += pi[(size_t)t * k + r];
+            }
+            xr[r] = sr;
+            xi[r] = si;
+          }
+        }
+        team.barrier();
+        if (tid == 0) {
+          // x = tau y, then x -= tau (x^H v) / 2 v
+          double xvr = 0.0, xvi = 0.0;  // (tau y)^H v
+          for (int r = 0; r < m; ++r) {
+            const double tr = ti.real() * xr[r] - ti.imag() * xi[r], tim = ti.real() * xi[r] + ti.imag() * xr[r];
+            xr[r] = tr;
+            xi[r] = tim;
+            xvr += tr * vr[r] + tim * vi[r];
+            xvi += tr * vi[r] - tim * vr[r];
+          }
+          const cd al2 = -0.5 * ti * cd(xvr, xvi);
+          for (int r = 0; r < m; ++r) {
+            xr[r] += al2.real() * vr[r] - al2.imag() * vi[r];
+            xi[r] += al2.real() * vi[r] + al2.imag() * vr[r];
+          }
+        }
+        team.barrier();
+        // A22 -= v x^H + x v^H, lower triangle, columns c = tid, tid + T, ...
+        for (int c = tid; c < m; c += T) {
+          double* ar = &Ar[(size_t)(o + c) * k + o];
+          double* ai = &Ai[(size_t)(o + c) * k + o];
+          const double xcr = xr[c], xci = -xi[c], vcr = vr[c], vci = -vi[c];  // conjugates
+          for (int r = c; r < m; ++r) {
+            ar[r] -= vr[r] * xcr - vi[r] * xci + xr[r] * vcr - xi[r] * vci;
+            ai[r] -= vr[r] * xci + vi[r] * xcr + xr[r] * vci + xi[r] * vcr;
+          }
+        }
+        team.barrier();
+      }
+      if (tid == 0) {
+        Ai[(size_t)o * k + o] = 0.0;
+        d[i] = Ar[(size_t)i * k + i];
+        tau[i] = ti;
       }
     }
-    Ai[(size_t)o * k + o] = 0.0;
-    d[i] = Ar[(size_t)i * k + i];
-    tau[i] = ti;
-  }
+  });
   d[k - 1] = Ar[(size_t)(k - 1) * k + k - 1];
   e[k - 1] = 0.0;
   // eigenvalues: implicit QL without vectors on a copy of (d, e)
@@ -257,8 +344,10 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V, int nvec) {
   // eigenvectors of the tridiagonal matrix for the nvec lowest eigenvalues
   std::vector<double> Z((size_t)k * nvec);
   if (!tri_inverse_iteration(k, d.data(), e.data(), dv.data(), nvec, Z.data())) return false;
+  Team bt(nvec >= 32 ? team_size(k) : 1);
+  bt.run([&](int tid) {
   std::vector<double> cr(k), ci(k);
-  for (int c = 0; c < nvec; ++c) {
+  for (int c = tid; c < nvec; c += bt.size()) {
     const double* z = &Z[(size_t)c * k];
     for (int r = 0; r < k; ++r) {
       cr[r] = z[r];
@@ -283,6 +372,7 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V, int nvec) {
     cd* col = V + (size_t)c * k;
     for (int r = 0; r < k; ++r) col[r] = cd(cr[r], ci[r]);
   }
+  });
   for (int c = 0; c < k; ++c)
     if (!std::isfinite(w[c])) return false;
   return true;
@@ -325,19 +415,69 @@ int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, s
   if (min_rel) *min_rel = mr;
   // R^-1 by back substitution, column by column (X upper, ld k)
   std::vector<cd> X((size_t)k * k, 0.0);
-  for (int c = 0; c < ke; ++c) {
-    cd* Xc = &X[(size_t)c * k];
-    Xc[c] = 1.0 / R[(size_t)c * k + c].real();
-    for (int r = c - 1; r >= 0; --r) {
-      cd sum = 0.0;
-      for (int l = r + 1; l <= c; ++l) sum += R[(size_t)l * k + r] * Xc[l];
-      Xc[r] = -sum / R[(size_t)r * k + r].real();
+  Team team(team_size(ke));
+  team.run([&](int tid) {
+    // columns from the last (the costliest) down, cyclically over the team
+    for (int c = ke - 1 - tid; c >= 0; c -= team.size()) {
+      cd* Xc = &X[(size_t)c * k];
+      Xc[c] = 1.0 / R[(size_t)c * k + c].real();
+      for (int r = c - 1; r >= 0; --r) {
+        cd sum = 0.0;
+        for (int l = r + 1; l <= c; ++l) sum += R[(size_t)l * k + r] * Xc[l];
+        Xc[r] = -sum / R[(size_t)r * k + r].real();
+      }
     }
-  }
+  });
   Cinv.assign((size_t)k * ke, 0.0);
   for (int c = 0; c < ke; ++c)
     for (int r = 0; r <= c; ++r) Cinv[(size_t)c * k + keep[r]] = X[(size_t)c * k + r];
   return ke;
+}
+
+// C (m x n) = op(A) B with op(A) = A (ta = 'N', A m x k) or A^H (ta = 'C',
+// A k x m), all column-major; columns of C split over a team when the
+// product is large.  The small dense products of the Rayleigh-Ritz step
+// (the second Cholesky-QR pass applied to the projected matrix, the Ritz
+// coefficients in the first-pass basis).
+void host_zgemm(char ta, int m, int n, int k, const cd* A, int lda, const cd* B, int ldb, cd* C, int ldc) {
+  const double vol = (double)m * n * k;
+  Team team(vol >= 4e6 ? team_size(192) : 1);
+  team.run([&](int tid) {
+    std::vector<double> cr(ta == 'N' ? m : 0), ci(ta == 'N' ? m : 0);
+    for (int c = tid; c < n; c += team.size()) {
+      const cd* b = B + (size_t)c * ldb;
+      cd* out = C + (size_t)c * ldc;
+      if (ta == 'N') {
+        // C[:, c] = sum_l A[:, l] B[l, c] (axpy form, split real / imaginary)
+        std::fill(cr.begin(), cr.end(), 0.0);
+        std::fill(ci.begin(), ci.end(), 0.0);
+        for (int l = 0; l < k; ++l) {
+          const double br = b[l].real(), bi = b[l].imag();
+          if (br == 0.0 && bi == 0.0) continue;
+          const double* a = reinterpret_cast<const double*>(A + (size_t)l * lda);
+          for (int r = 0; r < m; ++r) {
+            const double ar = a[2 * r], ai = a[2 * r + 1];
+            cr[r] += ar * br - ai * bi;
+            ci[r] += ar * bi + ai * br;
+          }
+        }
+        for (int r = 0; r < m; ++r) out[r] = cd(cr[r], ci[r]);
+      } else {
+        // C[r, c] = sum_l conj(A[l, r]) B[l, c]
+        const double* bb = reinterpret_cast<const double*>(b);
+        for (int r = 0; r < m; ++r) {
+          const double* a = reinterpret_cast<const double*>(A + (size_t)r * lda);
+          double sr = 0.0, si = 0.0;
+          for (int l = 0; l < k; ++l) {
+            const double ar = a[2 * l], ai = a[2 * l + 1], br = bb[2 * l], bi = bb[2 * l + 1];
+            sr += ar * br + ai * bi;
+            si += ar * bi - ai * br;
+          }
+          out[r] = cd(sr, si);
+        }
+      }
+    }
+  });
 }
 
 // A real symmetric matrix stays real through the Householder reduction and

@@ -175,6 +175,8 @@ void launch_r2(hipStream_t st, int n, double a, double h, const double* rho_n, c
 // QL.  Returns false on failure.
 // Only the first nvec (lowest) eigenvectors are formed (nvec < 0: all).
 bool eigh_lower(int k, const cd* A, double* w, cd* V, int nvec = -1);
+// C (m x n) = op(A) B, op = 'N' (A m x k) or 'C' (A^H, A k x m), column-major
+void host_zgemm(char ta, int m, int n, int k, const cd* A, int lda, const cd* B, int ldb, cd* C, int ldc);
 // Cholesky-QR coefficients with column dropping: given a Gram matrix G
 // (k x k Hermitian, lower triangle used) return the kept column list and
 // Cinv (k x ke) with V_eff = V * Cinv orthonormal.  drop_rel: a column is

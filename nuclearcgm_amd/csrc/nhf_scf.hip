@@ -645,18 +645,8 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
     if (pass2) {
       Pm.assign((size_t)k2 * k2, 0.0);
       std::vector<cd> T1((size_t)k1 * k2, 0.0);  // P1 C2
-      for (int c = 0; c < k2; ++c)
-        for (int l = 0; l < k1; ++l) {
-          cd cf = C2[(size_t)c * k1 + l];
-          if (cf == cd(0.0, 0.0)) continue;
-          for (int r = 0; r < k1; ++r) T1[(size_t)c * k1 + r] += P1[(size_t)l * k1 + r] * cf;
-        }
-      for (int c = 0; c < k2; ++c)
-        for (int r = 0; r < k2; ++r) {
-          cd s = 0.0;
-          for (int l = 0; l < k1; ++l) s += std::conj(C2[(size_t)r * k1 + l]) * T1[(size_t)c * k1 + l];
-          Pm[(size_t)c * k2 + r] = s;
-        }
+      host_zgemm('N', k1, k2, k1, P1.data(), k1, C2.data(), k1, T1.data(), k1);
+      host_zgemm('C', k2, k2, k1, C2.data(), k1, T1.data(), k1, Pm.data(), k2);
     } else {
       Pm.swap(P1);
     }
@@ -666,11 +656,7 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
     if (!eigh_lower(k2, Pm.data(), wk.data(), Ck.data(), rr)) return NHF_ESOLVER;
     std::vector<cd> Ctot((size_t)k1 * rr, 0.0);
     if (pass2) {
-      for (int c = 0; c < rr; ++c)
-        for (int l = 0; l < k2; ++l) {
-          cd cf = Ck[(size_t)c * k2 + l];
-          for (int r = 0; r < k1; ++r) Ctot[(size_t)c * k1 + r] += C2[(size_t)l * k1 + r] * cf;
-        }
+      host_zgemm('N', k1, rr, k2, C2.data(), k1, Ck.data(), k2, Ctot.data(), k1);
     } else {
       std::copy(Ck.begin(), Ck.begin() + (size_t)k1 * rr, Ctot.begin());
     }
@@ -688,20 +674,21 @@ static int gcg_dev(Problem& P, Work& Wk, int q, const Sp<R>* psi_in, Sp<R>* psi_
       HIPCHK(hipStreamSynchronize(st));
     }
     // locking: converged columns report the Rayleigh quotient Re(x^H A x)
+    // = Re(c^H Pm c) of their Ritz coefficients c (Pm c for all locked
+    // columns in one product)
     Lam.assign(M, 0.0);
-    for (int c = 0; c < rr; ++c) {
-      double lam = wk[c] - shift;
-      if (rn2[c] < prm.tol * prm.tol) {
-        const cd* cc = &Ck[(size_t)c * k2];
-        cd s = 0.0;
-        for (int j = 0; j < k2; ++j) {
-          cd t = 0.0;
-          for (int i = 0; i < k2; ++i) t += Pm[(size_t)j * k2 + i] * cc[j] * std::conj(cc[i]);
-          s += t;
-        }
-        lam = s.real() - shift;
-      }
-      Lam[c] = lam;
+    std::vector<int> lk;
+    for (int c = 0; c < rr; ++c)
+      if (rn2[c] < prm.tol * prm.tol) lk.push_back(c);
+    std::vector<cd> Cl((size_t)k2 * lk.size()), PCl((size_t)k2 * lk.size());
+    for (size_t j = 0; j < lk.size(); ++j)
+      std::copy(&Ck[(size_t)lk[j] * k2], &Ck[(size_t)lk[j] * k2] + k2, &Cl[j * k2]);
+    if (!lk.empty()) host_zgemm('N', k2, (int)lk.size(), k2, Pm.data(), k2, Cl.data(), k2, PCl.data(), k2);
+    for (int c = 0; c < rr; ++c) Lam[c] = wk[c] - shift;
+    for (size_t j = 0; j < lk.size(); ++j) {
+      cd s = 0.0;
+      for (int i = 0; i < k2; ++i) s += std::conj(Cl[j * k2 + i]) * PCl[j * k2 + i];
+      Lam[lk[j]] = s.real() - shift;
     }
     if (rr < M) HIPCHK(hipMemsetAsync(Xout + (size_t)rr * ld, 0, sizeof(Sp<R>) * ld * (M - rr), st));
     Wk.tph[4] += secs(tD, Clock::now());

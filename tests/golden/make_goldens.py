@@ -2,14 +2,15 @@
 and writes JSON fixtures of the results (energies, single-particle
 energies, observables, the E_int trace):
 
+    python tests/golden/make_goldens.py o16     # O-16 30^3 HF to convergence
     python tests/golden/make_goldens.py curve   # two points of the Mg/Si curve batch
     python tests/golden/make_goldens.py sn120   # Sn-120 48^3 HF+BCS to convergence
     python tests/golden/make_goldens.py pb208   # Pb-208 56^3 HFB, first iterations
 
-curve: Mg-24 (32^3) at beta2 = +0.4571 and Si-28 (40^3) at beta2 = -0.2286,
+curve: Mg-24 (32^3) at beta2 = +0.6286 and Si-28 (40^3) at beta2 = -0.0571,
 SkM*, 20+20 states, the curve's constraint list with a Q20 stiffness of
-0.1 MeV, each run to convergence (|dE| < 1e-6 MeV and constraints within
-1e-3).  sn120: to convergence.  pb208: the first PB208_ITERS HF iterations
+0.1 MeV, each from the Woods-Saxon start-up at its own beta2 and run to
+convergence (|dE| < 1e-6 MeV and constraints within 1e-3).  sn120: to convergence.  pb208: the first PB208_ITERS HF iterations
 (the BCS seed steps and the first HFB steps), whose E_int trace pins the
 densities, fields, pairing and energies of the full-size lattice.
 
@@ -29,12 +30,13 @@ from nuclearcgm_amd import config as C  # noqa: E402
 
 import bench  # noqa: E402
 
-CURVE_POINTS = (5, 9)  # indices into config.curve_batch(): Mg-24 beta2=+0.4571, Si-28 beta2=-0.2286
+CURVE_POINTS = (6, 10)  # indices into config.curve_batch(): Mg-24 beta2=+0.6286, Si-28 beta2=-0.0571
 PB208_ITERS = 5
 
 
-def run(cfg, label, extra=None):
-    guess = bench.make_guess(cfg)
+def run(cfg, label, extra=None, guess=None):
+    if guess is None:
+        guess = bench.make_guess(cfg)
     t = time.time()
     pn, pp, en, ep, res, trace = O.scf_run(cfg, *guess)
     s = res.summary()
@@ -49,13 +51,19 @@ def run(cfg, label, extra=None):
 
 
 def main(what):
-    if what == "curve":
+    if what == "o16":
+        cfg = C.workload("o16")
+        cfg.max_iter = 600
+        data = run(cfg, "O-16 30^3 HF")
+        path = os.path.join(HERE, "o16_oracle.json")
+    elif what == "curve":
         batch = C.curve_batch()
         out = []
         for i in CURVE_POINTS:
             label, beta, cfg = batch[i]
             cfg.max_iter = 600
-            out.append(run(cfg, label, {"index": i, "beta2_target": beta}))
+            g = O.ws_guess(cfg, beta2=beta)
+            out.append(run(cfg, label, {"index": i, "beta2_target": beta, "start": "Woods-Saxon"}, guess=g))
         path = os.path.join(HERE, "curve_oracle.json")
         data = out
     elif what == "sn120":

@@ -1,0 +1,81 @@
+"""The workloads at their full sizes on the GPU, against the FP64 oracle's
+fixtures (tests/golden/make_goldens.py):
+
+* O-16, 30^3, 12+12 states (the plain-method workload): the FP64 path
+  against the oracle's converged energies and single-particle energies;
+  the accelerated path (FP32 states, split-BF16 block products, Broyden
+  mixing, 8-step Jacobi-PCG) within 1 keV of the oracle.
+
+The device's FP64 sums use atomics (not bitwise reproducible) and the HF
+iteration amplifies rounding-level differences before converging, so the
+FP64 comparisons are made at the converged fixed point with the
+tolerances of the oracle's own run-to-run spread there."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import bench
+from nuclearcgm_amd import config as C
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def gold(name):
+    with open(os.path.join(GOLD, name)) as fh:
+        return json.load(fh)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from nuclearcgm_amd import hip
+
+    c = hip.Context(0)
+    yield c
+    c.close()
+
+
+def accelerated(c):
+    """Mixed precision + Broyden mixing + 8-step Jacobi-PCG W-block (the
+    learned W-block is covered by the preconditioner tests)."""
+    c.mixing = C.MIX_BROYDEN
+    c.wstep, c.gcg.steps = C.W_JACOBI_CG, 8
+    return c
+
+
+def check_fp64(res, g, cfg, tol_e):
+    """Converged FP64 run against the oracle's: energies, radii, and the
+    single-particle energies -- the occupied ones (the lowest N and Z) to
+    1e-5 MeV, the whole block to 1e-3 MeV (empty levels do not enter E_tot,
+    so they are still settling when the energy criterion stops a run; in a
+    near-degenerate empty shell they differ by ~1e-4 MeV between runs)."""
+    r, en, ep = res[4], res[2], res[3]
+    print(f"E_int gpu {r.terms[C.E_INT]:.9f} oracle {g['E_int']:.9f} ({r.iterations} / {g['iterations']} it)")
+    assert r.converged and g["converged"]
+    assert abs(r.terms[C.E_INT] - g["E_int"]) < tol_e
+    assert abs(r.terms[C.E_HF] - g["E_HF"]) < 10 * tol_e
+    for eps, ref, nocc in ((en, g["eps_n"], cfg.A - cfg.Z), (ep, g["eps_p"], cfg.Z)):
+        occ, allst = bench.spe_errors(eps, ref, nocc)
+        print(f"  max |d eps| occupied {occ:.2e}, all {allst:.2e} MeV")
+        assert occ < 1e-5 and allst < 1e-3
+    assert abs(np.sqrt(r.r2_n) - g["rms_n"]) < 1e-6 and abs(np.sqrt(r.r2_p) - g["rms_p"]) < 1e-6
+
+
+@pytest.mark.parametrize("name,fixture,tol_e", [("o16", "o16_oracle.json", 1e-5)])
+def test_fp64_and_accelerated_match_oracle(ctx, name, fixture, tol_e):
+    g = gold(fixture)
+    c64 = C.workload(name, precision=C.PREC_FP64)
+    c64.max_iter = 600
+    guess = bench.make_guess(c64)
+    check_fp64(ctx.problem(c64).scf_run(*guess), g, c64, tol_e)
+    cm = accelerated(C.workload(name, precision=C.PREC_MIXED))
+    cm.max_iter = 400
+    r = ctx.problem(cm).scf_run(*guess)
+    t, k = bench.time_to_window(r[5], g["E_int"])
+    N, Z = cm.A - cm.Z, cm.Z
+    print(f"accelerated: {r[4].iterations} it, E_int {r[4].terms[C.E_INT]:.7f}, 1 keV after {k} it ({t:.3f} s)")
+    assert k > 0 and abs(r[4].terms[C.E_INT] - g["E_int"]) < 1e-3
+    assert bench.spe_errors(r[2], g["eps_n"], N)[0] < 2e-3 and bench.spe_errors(r[3], g["eps_p"], Z)[0] < 2e-3
