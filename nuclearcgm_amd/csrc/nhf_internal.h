@@ -81,9 +81,13 @@ int gram_chunks(size_t N3);
 // Y (ncols) = V (k cols) * C (k x ncols complex double, column-major),
 // Y += ... when accumulate; when AV != null also accumulates
 // |AV c - w_c V c|^2 into rn2[c] (zeroed by the caller).
+// frag (optional, FP32 states without residuals): workspace of
+// rotate_frag_count(k, ncols) float4 for the matrix-core kernel
 template <typename R>
 void launch_rotate(hipStream_t st, const Sp<R>* V, size_t ldv, int k, const double2* C, int ncols, Sp<R>* Y,
-                   size_t ldy, size_t N3, const Sp<R>* AV, const double* w, double* rn2, int accumulate);
+                   size_t ldy, size_t N3, const Sp<R>* AV, const double* w, double* rn2, int accumulate,
+                   float4* frag = nullptr);
+size_t rotate_frag_count(int k, int ncols);
 
 template <typename R>
 void launch_axpby_cols(hipStream_t st, const Sp<R>* X, Sp<R>* Y, size_t N3, size_t ld, int ncols, const double* a,

@@ -1280,9 +1280,179 @@ __global__ __launch_bounds__(BS) void k_rotate(const Sp<R>* __restrict__ V, size
   if (RES) block_atomic_add<RTN>(res, rn2 + c0);
 }
 
+// The rotation of FP32 states on the FP32 matrix cores
+// (v_mfma_f32_16x16x4f32: exact FP32 products, FP32 accumulation, as the
+// VALU kernel).  Real form with rows j = 2p + s (point p, spin s) and K
+// index 2l + part (state l, part 0 = re, 1 = im):
+//   Re Y[j, c] = sum_K Are[K, c] B[K, j],  Im Y[j, c] = sum_K Aim[K, c] B[K, j],
+//   B[2l, j] = Re V[j, l], B[2l + 1, j] = Im V[j, l],
+//   Are[2l, c] = Re C[l, c], Are[2l + 1, c] = -Im C[l, c],
+//   Aim[2l, c] = Im C[l, c], Aim[2l + 1, c] = Re C[l, c].
+// The MFMA computes D (16 x 16) += A (16 x 4) B (4 x 16) with lane L holding
+// A[L % 16][L / 16], B[L / 16][L % 16] and D[4 (L / 16) + q][L % 16]; here A
+// is Are^T or Aim^T (rows = output columns) and the columns of D are rows j.
+// A workgroup (4 waves) owns 64 points (128 rows) and RCT column tiles of 16;
+// the states go in chunks of 8 (16 K): the V tile (8 columns x 64 points) is
+// loaded with 16-byte loads into LDS as B[j][2 l + part] (row stride
+// ROT_RS floats, double-buffered), so a lane reads its four K values for
+// the chunk's four MFMAs with one 16-byte read (lane group g takes K = 4g + t
+// in MFMA t).  The A operands are laid out per chunk, column tile and lane
+// by k_rot_frags (one float4 per re / im form).
+constexpr int ROT_RS = 20;
+
+__global__ void k_rot_frags(const double2* __restrict__ C, int k, int ncols, int nt, int nu, float4* __restrict__ F) {
+  const int e = blockIdx.x * BS + threadIdx.x;  // (u, cb, lane)
+  if (e >= nu * nt * 64) return;
+  const int lane = e & 63, cb = (e >> 6) % nt, u = (e >> 6) / nt;
+  const int c = 16 * cb + (lane & 15), g = lane >> 4;
+  float re[4], im[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int l = 8 * u + 2 * g + t / 2;
+    double2 v = make_double2(0.0, 0.0);
+    if (l < k && c < ncols) v = C[(size_t)c * k + l];
+    re[t] = (t & 1) ? (float)-v.y : (float)v.x;
+    im[t] = (t & 1) ? (float)v.x : (float)v.y;
+  }
+  F[(((size_t)u * nt + cb) * 2) * 64 + lane] = make_float4(re[0], re[1], re[2], re[3]);
+  F[(((size_t)u * nt + cb) * 2 + 1) * 64 + lane] = make_float4(im[0], im[1], im[2], im[3]);
+}
+
+template <int RCT>
+__global__ __launch_bounds__(BS) void k_rotate_mfma(const Sp<float>* __restrict__ V, size_t ldv, int k,
+                                                    const float4* __restrict__ F, int nt, int ncols,
+                                                    Sp<float>* __restrict__ Y, size_t ldy, size_t N3, int accumulate) {
+  __shared__ __align__(16) float Bl[2][128 * ROT_RS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, jl = lane & 15;
+  const size_t p0 = (size_t)blockIdx.x * 64;
+  const int cb0 = blockIdx.y * RCT;
+  const int nu = (k + 7) / 8;
+  // staging: thread t loads state 8u + t % 8 at points p0 + t / 8 and
+  // p0 + 32 + t / 8 (8 columns x 8 points: eight 128-byte lines per wave)
+  const int sl = tid & 7, sp = tid >> 3;
+  auto load = [&](int u, float4 (&r)[2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const size_t p = p0 + sp + 32 * h;
+      const int l = 8 * u + sl;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (p < N3 && l < k) v = *reinterpret_cast<const float4*>(V + (size_t)l * ldv + p);
+      r[h] = v;
+    }
+  };
+  auto store = [&](int buf, const float4 (&r)[2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int p = sp + 32 * h;
+      float* b = &Bl[buf][0];
+      *reinterpret_cast<float2*>(b + (2 * p) * ROT_RS + 2 * sl) = make_float2(r[h].x, r[h].y);
+      *reinterpret_cast<float2*>(b + (2 * p + 1) * ROT_RS + 2 * sl) = make_float2(r[h].z, r[h].w);
+    }
+  };
+  gf32x4 dre[2][RCT], dim[2][RCT];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < RCT; ++b) {
+      dre[a][b] = gf32x4{0.f, 0.f, 0.f, 0.f};
+      dim[a][b] = gf32x4{0.f, 0.f, 0.f, 0.f};
+    }
+  float4 r[2];
+  load(0, r);
+  store(0, r);
+  __syncthreads();
+  for (int u = 0; u < nu; ++u) {
+    const int cur = u & 1;
+    if (u + 1 < nu) load(u + 1, r);
+    float4 bv[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int j = (wave * 2 + a) * 16 + jl;
+      bv[a] = *reinterpret_cast<const float4*>(&Bl[cur][j * ROT_RS + 4 * g]);
+    }
+#pragma unroll
+    for (int b = 0; b < RCT; ++b) {
+      const int cb = cb0 + b;
+      const bool on = cb < nt;
+      const size_t ai = (((size_t)u * nt + (on ? cb : 0)) * 2) * 64 + lane;
+      const float4 are = on ? F[ai] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 aim = on ? F[ai + 64] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        dre[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(are.x, bv[a].x, dre[a][b], 0, 0, 0);
+        dim[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aim.x, bv[a].x, dim[a][b], 0, 0, 0);
+        dre[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(are.y, bv[a].y, dre[a][b], 0, 0, 0);
+        dim[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aim.y, bv[a].y, dim[a][b], 0, 0, 0);
+        dre[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(are.z, bv[a].z, dre[a][b], 0, 0, 0);
+        dim[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aim.z, bv[a].z, dim[a][b], 0, 0, 0);
+        dre[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(are.w, bv[a].w, dre[a][b], 0, 0, 0);
+        dim[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aim.w, bv[a].w, dim[a][b], 0, 0, 0);
+      }
+    }
+    // the other image was last read in the previous chunk, and every thread
+    // passed the barrier that ended it
+    if (u + 1 < nu) store(cur ^ 1, r);
+    __syncthreads();
+  }
+  // lane holds D[c = 16 cb + 4 g + q][j = 16 (2 wave + a) + jl]
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int j = (wave * 2 + a) * 16 + jl;
+    const size_t p = p0 + (j >> 1);
+    if (p >= N3) continue;
+    const int s = j & 1;
+#pragma unroll
+    for (int b = 0; b < RCT; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = 16 * (cb0 + b) + 4 * g + q;
+        if (c < ncols) {
+          float2* yp = reinterpret_cast<float2*>(reinterpret_cast<float*>(Y + (size_t)c * ldy + p) + 2 * s);
+          float2 o = make_float2(dre[a][b][q], dim[a][b][q]);
+          if (accumulate) {
+            const float2 old = *yp;
+            o.x += old.x;
+            o.y += old.y;
+          }
+          *yp = o;
+        }
+      }
+  }
+}
+
+size_t rotate_frag_count(int k, int ncols) {
+  return (size_t)((k + 7) / 8) * ((ncols + 15) / 16) * 2 * 64;
+}
+
+// NHF_ROT=valu: the VALU rotation kernel for FP32 states too
+static bool rot_valu() {
+  static bool v = [] {
+    const char* e = std::getenv("NHF_ROT");
+    return e && std::strcmp(e, "valu") == 0;
+  }();
+  return v;
+}
+
 template <typename R>
 void launch_rotate(hipStream_t st, const Sp<R>* V, size_t ldv, int k, const double2* C, int ncols, Sp<R>* Y,
-                   size_t ldy, size_t N3, const Sp<R>* AV, const double* w, double* rn2, int accumulate) {
+                   size_t ldy, size_t N3, const Sp<R>* AV, const double* w, double* rn2, int accumulate,
+                   float4* frag) {
+  if (std::is_same<R, float>::value && !AV && frag && !block_fp64() && !rot_valu()) {
+    const int nt = (ncols + 15) / 16, nu = (k + 7) / 8;
+    const unsigned nfe = (unsigned)(nu * nt * 64);
+    hipLaunchKernelGGL(k_rot_frags, dim3((nfe + BS - 1) / BS), dim3(BS), 0, st, C, k, ncols, nt, nu, frag);
+    const unsigned gx = (unsigned)((N3 + 63) / 64);
+    const Sp<float>* Vf = reinterpret_cast<const Sp<float>*>(V);
+    Sp<float>* Yf = reinterpret_cast<Sp<float>*>(Y);
+    if (nt > 4)  // three column tiles per workgroup: fewer re-reads of V
+      hipLaunchKernelGGL((k_rotate_mfma<3>), dim3(gx, (unsigned)((nt + 2) / 3)), dim3(BS), 0, st, Vf, ldv, k, frag, nt,
+                         ncols, Yf, ldy, N3, accumulate);
+    else
+      hipLaunchKernelGGL((k_rotate_mfma<2>), dim3(gx, (unsigned)((nt + 1) / 2)), dim3(BS), 0, st, Vf, ldv, k, frag, nt,
+                         ncols, Yf, ldy, N3, accumulate);
+    return;
+  }
   if (std::is_same<R, float>::value && !AV && !block_fp64()) {
     constexpr int RF = 16;
     dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)((ncols + RF - 1) / RF));
@@ -1301,9 +1471,9 @@ void launch_rotate(hipStream_t st, const Sp<R>* V, size_t ldv, int k, const doub
                        accumulate);
 }
 template void launch_rotate<double>(hipStream_t, const Sp<double>*, size_t, int, const double2*, int, Sp<double>*,
-                                    size_t, size_t, const Sp<double>*, const double*, double*, int);
+                                    size_t, size_t, const Sp<double>*, const double*, double*, int, float4*);
 template void launch_rotate<float>(hipStream_t, const Sp<float>*, size_t, int, const double2*, int, Sp<float>*, size_t,
-                                   size_t, const Sp<float>*, const double*, double*, int);
+                                   size_t, const Sp<float>*, const double*, double*, int, float4*);
 
 // ------------------------------------------------------- column utilities
 template <typename R>

@@ -60,6 +60,7 @@ struct Work {
   hipStream_t st = nullptr;
   void *V, *V1, *AV, *invd;
   double2 *gpart, *gout;
+  float4* rfrag;  // matrix-core operands of the FP32 rotation
   double *cg_scal, *thr, *rhs2, *lam_shift, *rn2, *norms, *scal_col;
   int *done, *its, *clist;
   // live timing (HIP events on this stream) of the two kernel classes that
@@ -306,6 +307,7 @@ static int setup_problem(Problem& P) {
     W.invd = tmp;
     CHK(P.alloc(W.gpart, (size_t)P.gchunks * P.kmax * P.kmax));
     CHK(P.alloc(W.gout, (size_t)P.kmax * P.kmax));
+    CHK(P.alloc(W.rfrag, rotate_frag_count(P.kmax, P.kmax)));
     CHK(P.alloc(W.cg_scal, (size_t)(P.steps_cap + 1) * P.Mmax * CG_NS));
     CHK(P.alloc(W.thr, P.Mmax));
     CHK(P.alloc(W.rhs2, P.Mmax));
@@ -399,7 +401,7 @@ static int rotate_dev(Problem& P, Work& W, const Sp<R>* V, int k, const std::vec
     HIPCHK(hipMemsetAsync(W.rn2, 0, sizeof(double) * (ncols + 8), W.st));
   }
   launch_rotate<R>(W.st, V, P.N3, k, W.gout, ncols, Y, P.N3, P.N3, AV, AV ? W.lam_shift : nullptr,
-                   AV ? W.rn2 : nullptr, accumulate ? 1 : 0);
+                   AV ? W.rn2 : nullptr, accumulate ? 1 : 0, W.rfrag);
   HIPCHK(hipStreamSynchronize(W.st));  // the host coefficients must outlive the copy
   return NHF_OK;
 }
