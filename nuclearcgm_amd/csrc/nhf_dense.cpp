@@ -15,6 +15,8 @@
 #include <numeric>
 #include <thread>
 
+#include <sched.h>
+
 #include "nhf_internal.h"
 
 namespace nhf {
@@ -22,11 +24,40 @@ namespace nhf {
 // A team of host threads for the O(k^3) loops of large bases (k >= 192, e.g.
 // the 480-column basis of Pb-208): the caller is member 0, the others are
 // started by run() and joined at its end; barrier() is a sense-reversing
-// spin barrier between the phases of one Householder step.  With one member
-// everything runs inline.
+// barrier between the phases of one Householder step that spins briefly and
+// then yields.  Several solvers run at once (two species per problem,
+// several problems in a batch), so members come from a process-wide budget
+// of the CPUs this process may use: a team gets at most what is free when it
+// starts, and at least its caller.  With one member everything runs inline.
+static int cpus_allowed() {
+  static const int n = [] {
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) return std::max(1, (int)CPU_COUNT(&set));
+    return std::max(1, (int)std::thread::hardware_concurrency());
+  }();
+  return n;
+}
+static std::atomic<int> g_team_threads{0};  // extra members currently running
+
 class Team {
  public:
-  explicit Team(int n) : n_(std::max(1, n)) {}
+  explicit Team(int want) {
+    int extra = std::max(0, want - 1);
+    if (extra > 0) {
+      // take at most the CPUs not already held by other teams (one per
+      // concurrent solver is left for the callers themselves)
+      const int cap = std::max(0, cpus_allowed() / 2 - g_team_threads.load(std::memory_order_relaxed));
+      extra = std::min(extra, cap);
+      g_team_threads.fetch_add(extra, std::memory_order_relaxed);
+    }
+    n_ = 1 + extra;
+  }
+  ~Team() {
+    if (n_ > 1) g_team_threads.fetch_sub(n_ - 1, std::memory_order_relaxed);
+  }
+  Team(const Team&) = delete;
+  Team& operator=(const Team&) = delete;
   int size() const { return n_; }
   template <class F>
   void run(F&& f) {
@@ -43,25 +74,31 @@ class Team {
       cnt_.store(0, std::memory_order_relaxed);
       gen_.store(g + 1, std::memory_order_release);
     } else {
-      while (gen_.load(std::memory_order_acquire) == g) __builtin_ia32_pause();
+      int spins = 0;
+      while (gen_.load(std::memory_order_acquire) == g) {
+        if (++spins < 4096)
+          __builtin_ia32_pause();
+        else
+          std::this_thread::yield();
+      }
     }
   }
 
  private:
-  int n_;
+  int n_ = 1;
   std::atomic<int> cnt_{0}, gen_{0};
 };
 
-// members for a k x k problem: one below k = 192, else up to 8 (half the
-// host's hardware threads: the two species solve concurrently); the
-// environment variable NHF_HOST_THREADS sets the count for large problems
+// members wanted for a k x k problem: one below k = 192, else up to 8 (a
+// quarter of the CPUs this process may use: the two species solve
+// concurrently, and problems may run side by side); the environment
+// variable NHF_HOST_THREADS sets the count for large problems
 static int team_size(int k) {
   if (k < 192) return 1;
   static const int n = [] {
     const char* e = std::getenv("NHF_HOST_THREADS");
     if (e && *e) return std::max(1, std::min(64, std::atoi(e)));
-    const int hw = (int)std::thread::hardware_concurrency();
-    return std::max(1, std::min(8, hw / 2));
+    return std::max(1, std::min(8, cpus_allowed() / 4));
   }();
   return n;
 }
