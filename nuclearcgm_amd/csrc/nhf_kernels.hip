@@ -1754,22 +1754,28 @@ __device__ __forceinline__ double2 sig_prod(const C4& a, int l, const C4& b) {
   return make_double2(re, im);
 }
 
+constexpr int DPTS = 64;           // points per workgroup of k_densities
+constexpr int DGRP = BS / DPTS;    // state groups (one wavefront each)
 template <typename R>
 __global__ __launch_bounds__(BS) void k_densities(const Sp<R>* __restrict__ psi, size_t ld, int m,
                                                   const double* __restrict__ occ, int n, double h, double mu,
                                                   int first, double* __restrict__ rho, double* __restrict__ tau,
                                                   double* __restrict__ J9, double* __restrict__ divJ) {
+  // a workgroup owns DPTS points; its four wavefronts take the states
+  // c = wave, wave + 4, ... (four times the waves in flight of one thread per
+  // point), and the partial sums meet in LDS
   const size_t N3 = (size_t)n * n * n;
-  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
-  if (p >= N3) return;
-  int i, j, k;
-  decompose(p, n, i, j, k);
+  const int lane = threadIdx.x % DPTS, grp = threadIdx.x / DPTS;
+  const size_t p = (size_t)blockIdx.x * DPTS + lane;
+  const bool on = p < N3;
+  int i = 0, j = 0, k = 0;
+  if (on) decompose(p, n, i, j, k);
   const int pos[3] = {i, j, k};
   const size_t stride[3] = {1, (size_t)n, (size_t)n * n};
   double r = 0.0, t = 0.0, dj = 0.0, J[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) J[q] = 0.0;
-  for (int c = 0; c < m; ++c) {
+  for (int c = grp; on && c < m; c += DGRP) {
     const double w = occ[c];
     if (w == 0.0) continue;
     const Sp<R>* __restrict__ col = psi + (size_t)c * ld;
@@ -1800,6 +1806,27 @@ __global__ __launch_bounds__(BS) void k_densities(const Sp<R>* __restrict__ psi,
     // div J = -2 [Im(g_y^+ sz g_x) + Im(g_z^+ sx g_y) + Im(g_x^+ sy g_z)]
     dj += -2.0 * w * (sig_prod(g[1], 2, g[0]).y + sig_prod(g[2], 0, g[1]).y + sig_prod(g[0], 1, g[2]).y);
   }
+  // sum of the DGRP partial sums of each point (group 0 keeps the result)
+  __shared__ double part[DGRP - 1][12][DPTS];
+  if (grp > 0) {
+    double* o = &part[grp - 1][0][lane];
+    o[0] = r;
+    o[DPTS] = t;
+    o[2 * DPTS] = dj;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) o[(3 + q) * DPTS] = J[q];
+  }
+  __syncthreads();
+  if (grp > 0 || !on) return;
+#pragma unroll
+  for (int g2 = 0; g2 < DGRP - 1; ++g2) {
+    const double* o = &part[g2][0][lane];
+    r += o[0];
+    t += o[DPTS];
+    dj += o[2 * DPTS];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) J[q] += o[(3 + q) * DPTS];
+  }
   if (first) {  // no read of the old values (the buffer may be uninitialised)
     rho[p] = r;
     tau[p] = t;
@@ -1819,8 +1846,8 @@ template <typename R>
 void launch_densities(hipStream_t st, const Sp<R>* psi, size_t ld, int m, const double* occ, int n, double h,
                       double mu, int first, double* rho, double* tau, double* J9, double* divJ) {
   size_t N3 = (size_t)n * n * n;
-  hipLaunchKernelGGL((k_densities<R>), dim3((unsigned)((N3 + BS - 1) / BS)), dim3(BS), 0, st, psi, ld, m, occ, n, h,
-                     mu, first, rho, tau, J9, divJ);
+  hipLaunchKernelGGL((k_densities<R>), dim3((unsigned)((N3 + DPTS - 1) / DPTS)), dim3(BS), 0, st, psi, ld, m, occ, n,
+                     h, mu, first, rho, tau, J9, divJ);
 }
 template void launch_densities<double>(hipStream_t, const Sp<double>*, size_t, int, const double*, int, double, double,
                                        int, double*, double*, double*, double*);
