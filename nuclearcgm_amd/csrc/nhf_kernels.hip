@@ -694,33 +694,52 @@ __global__ __launch_bounds__(BS) void k_gram(const Sp<R>* __restrict__ A, size_t
 // sums the chunk partials; with a lower-triangle tile edge `lower` > 0
 // (Hermitian result) the entries of skipped upper tiles are taken as the
 // conjugate of their mirror.  Workgroup: 32 entries x 8 groups of chunks.
-__global__ __launch_bounds__(BS) void k_reduce_parts(const double2* __restrict__ part, int nchunk, size_t cnt, int ka,
-                                                     int lower, double2* __restrict__ C) {
-  __shared__ double2 sh[BS / 32][32];
-  const int le = threadIdx.x % 32, cg = threadIdx.x / 32;
-  const size_t e = (size_t)blockIdx.x * 32 + le;
-  double sr = 0.0, si = 0.0;
+// Sum of the per-chunk partial products: C[e] = sum_c part[c][e] in FP64.  A
+// workgroup takes RED_EPB entries and spreads their chunks over
+// BS / RED_EPB thread groups (four independent chains per thread), so that
+// even a 32 x 32 product (1024 entries) keeps 128 workgroups busy.  Entries
+// (i, j) with i / edge < j / edge (edge > 0: Hermitian product whose upper
+// tiles were not computed) are the conjugate of their mirror.
+constexpr int RED_EPB = 8;
+template <typename T>
+__global__ __launch_bounds__(BS) void k_reduce_chunks(const T* __restrict__ part, int nchunk, size_t cnt, int ka,
+                                                      int edge, double2* __restrict__ C) {
+  constexpr int G = BS / RED_EPB;
+  __shared__ double2 sh[G][RED_EPB];
+  const int le = threadIdx.x % RED_EPB, cg = threadIdx.x / RED_EPB;
+  const size_t e = (size_t)blockIdx.x * RED_EPB + le;
+  double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
   bool mirror = false;
   if (e < cnt) {
     const int i = (int)(e % ka), j = (int)(e / ka);
-    mirror = lower > 0 && (i / lower) < (j / lower);
+    mirror = edge > 0 && (i / edge) < (j / edge);
     const size_t src = mirror ? (size_t)i * ka + j : e;
-    for (int c = cg; c < nchunk; c += BS / 32) {
-      const double2 v = part[(size_t)c * cnt + src];
-      sr += v.x;
-      si += v.y;
+    int ch = cg;
+    for (; ch + 3 * G < nchunk; ch += 4 * G) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const T v = part[(size_t)(ch + u * G) * cnt + src];
+        sr[u] += (double)v.x;
+        si[u] += (double)v.y;
+      }
+    }
+    for (; ch < nchunk; ch += G) {
+      const T v = part[(size_t)ch * cnt + src];
+      sr[0] += (double)v.x;
+      si[0] += (double)v.y;
     }
   }
-  sh[cg][le] = make_double2(sr, si);
+  sh[cg][le] = make_double2((sr[0] + sr[1]) + (sr[2] + sr[3]), (si[0] + si[1]) + (si[2] + si[3]));
   __syncthreads();
   if (cg != 0 || e >= cnt) return;
-  for (int k = 1; k < BS / 32; ++k) {
-    sr += sh[k][le].x;
-    si += sh[k][le].y;
+  double tr = 0.0, ti = 0.0;
+  for (int k = 0; k < G; ++k) {
+    tr += sh[k][le].x;
+    ti += sh[k][le].y;
   }
-  C[e] = mirror ? make_double2(sr, -si) : make_double2(sr, si);
+  C[e] = mirror ? make_double2(tr, -ti) : make_double2(tr, ti);
 }
-static unsigned reduce_grid(size_t cnt) { return (unsigned)((cnt + 31) / 32); }
+static unsigned reduce_grid(size_t cnt) { return (unsigned)((cnt + RED_EPB - 1) / RED_EPB); }
 
 // ---- split-precision MFMA Gram of FP32 states (v_mfma_f32_16x16x16_bf16).
 // Each FP32 operand is split three ways, x = h + m + l with h = bf16(x),
@@ -893,7 +912,7 @@ __global__ __launch_bounds__(BS) void k_gram_mfma(const Sp<float>* __restrict__ 
 // lower tile triangle, as nine row triples.  FP32 accumulators are flushed
 // into FP64 sums every two slabs; the chunk results are stored as FP32
 // partials (a rounding of 2^-24 of each chunk sum, independent across
-// chunks) and summed in FP64 by k_reduce_parts_f.
+// chunks) and summed in FP64 by k_reduce_chunks.
 typedef __bf16 gbf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int gu32x4 __attribute__((ext_vector_type(4)));
 constexpr int GS = 96;   // super-tile edge
@@ -1098,47 +1117,6 @@ __global__ __launch_bounds__(GBS) void k_gram_lds(const Sp<float>* __restrict__ 
     }
   }
 }
-// sums FP32 chunk partials in FP64 (four independent chains per thread);
-// entries (i, j) with i / edge < j / edge (edge > 0: Hermitian product,
-// upper tiles not computed) are the conjugate of their mirror
-__global__ __launch_bounds__(BS) void k_reduce_parts_f(const float2* __restrict__ part, int nchunk, size_t cnt,
-                                                       int ka, int edge, double2* __restrict__ C) {
-  __shared__ double2 sh[BS / 32][32];
-  const int le = threadIdx.x % 32, cg = threadIdx.x / 32;
-  const size_t e = (size_t)blockIdx.x * 32 + le;
-  double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
-  bool mirror = false;
-  if (e < cnt) {
-    const int i = (int)(e % ka), j = (int)(e / ka);
-    mirror = edge > 0 && (i / edge) < (j / edge);
-    const size_t src = mirror ? (size_t)i * ka + j : e;
-    constexpr int G = BS / 32;
-    int ch = cg;
-    for (; ch + 3 * G < nchunk; ch += 4 * G) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float2 v = part[(size_t)(ch + u * G) * cnt + src];
-        sr[u] += v.x;
-        si[u] += v.y;
-      }
-    }
-    for (; ch < nchunk; ch += G) {
-      const float2 v = part[(size_t)ch * cnt + src];
-      sr[0] += v.x;
-      si[0] += v.y;
-    }
-  }
-  sh[cg][le] = make_double2((sr[0] + sr[1]) + (sr[2] + sr[3]), (si[0] + si[1]) + (si[2] + si[3]));
-  __syncthreads();
-  if (cg != 0 || e >= cnt) return;
-  double tr = sh[0][le].x, ti = sh[0][le].y;
-  for (int k = 1; k < BS / 32; ++k) {
-    tr += sh[k][le].x;
-    ti += sh[k][le].y;
-  }
-  C[e] = mirror ? make_double2(tr, -ti) : make_double2(tr, ti);
-}
-
 // chunk length (points) for the split Gram: enough workgroups to fill the
 // device (about 1024), a multiple of 16, at least 256 points
 static int gram_mfma_chunk(size_t N3, int blocks) {
@@ -1165,7 +1143,7 @@ void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>
     float2* pf = reinterpret_cast<float2*>(part);
     LAUNCH_EV(e0, none, k_gram_lds, dim3(nsi * nsj, (unsigned)nc), dim3(GBS), 0, st, (const Sp<float>*)A, lda, ka,
               (const Sp<float>*)B, ldb, kb, N3, (int)cp, pf, lower);
-    LAUNCH_EV(none, e1, k_reduce_parts_f, dim3(reduce_grid(cnt)), dim3(BS), 0, st, pf, (int)nc, cnt, ka,
+    LAUNCH_EV(none, e1, k_reduce_chunks<float2>, dim3(reduce_grid(cnt)), dim3(BS), 0, st, pf, (int)nc, cnt, ka,
               lower ? 16 : 0, C);
     return;
   }
@@ -1175,15 +1153,16 @@ void launch_gram(hipStream_t st, const Sp<R>* A, size_t lda, int ka, const Sp<R>
     const int nc = (int)((N3 + cp - 1) / cp);
     LAUNCH_EV(e0, none, k_gram_mfma, dim3(nbi * nbj, nc), dim3(BS), 0, st, (const Sp<float>*)A, lda, ka,
               (const Sp<float>*)B, ldb, kb, N3, cp, part, lower);
-    LAUNCH_EV(none, e1, k_reduce_parts, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, nc, cnt, ka, lower ? GB : 0,
-              C);
+    LAUNCH_EV(none, e1, k_reduce_chunks<double2>, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, nc, cnt, ka,
+              lower ? GB : 0, C);
     (void)nchunk;
     return;
   }
   int nti = (ka + GT - 1) / GT, ntj = (kb + GT - 1) / GT;
   const int ncd = std::min(nchunk, (int)((N3 + GRAM_CHUNK - 1) / GRAM_CHUNK));
   LAUNCH_EV(e0, none, (k_gram<R>), dim3(nti * ntj, ncd), dim3(BS), 0, st, A, lda, ka, B, ldb, kb, N3, part, lower);
-  LAUNCH_EV(none, e1, k_reduce_parts, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, ncd, cnt, ka, lower ? GT : 0, C);
+  LAUNCH_EV(none, e1, k_reduce_chunks<double2>, dim3(reduce_grid(cnt)), dim3(BS), 0, st, part, ncd, cnt, ka,
+            lower ? GT : 0, C);
 }
 template void launch_gram<double>(hipStream_t, const Sp<double>*, size_t, int, const Sp<double>*, size_t, int, size_t,
                                   double2*, int, double2*, int, const LaunchEvents*);
