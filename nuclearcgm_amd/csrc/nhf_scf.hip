@@ -172,6 +172,9 @@ struct Problem {
   int* dcols;
   // host state
   bool have = false;
+  // constraints within constraints_tol at the last energy evaluation (true
+  // without constraints): Broyden mixing waits for it
+  bool cons_ok = true;
   double tcommon = 0.0, tenergy = 0.0;  // wall time of the shared update / energies
   BcsState bcs[2];
   // two-basis HFB state per species (M x M, column-major) and the pairing
@@ -745,7 +748,7 @@ static int poisson_dev(Problem& P, hipStream_t st, const double* rho_p, double* 
       launch_dst_precond(st, P.pr, P.pz, P.ptmp, P.pS, P.n, P.h);
       launch_dot(st, P.pr, P.pz, N3, sc + 2);                          // rz
       launch_pcg_dir(st, P.pz, P.pp, N3, P.pscal, it);                  // beta = rz / rz_prev
-      if (it % 4 == 0 || it == maxit) {
+      if (it % 2 == 0 || it == maxit) {  // residual read back every second iteration
         double rr;
         HIPCHK(hipMemcpyAsync(&rr, sc + 1, sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1180,7 +1183,16 @@ static int update_common(Problem& P, Sp<R>* const psi[2], const std::vector<doub
   for (int q = 0; q < 2; ++q)
     HIPCHK(hipMemcpyAsync(P.occ[q], occ[q].data(), sizeof(double) * P.M[q], hipMemcpyHostToDevice, st));
   const int first = P.have ? 0 : 1;
-  if (c.mixing == NHF_MIX_BROYDEN && P.have) {
+  // Broyden mixing once the constraints hold: while a constraint multiplier
+  // still moves every iteration, the map being mixed changes under the
+  // secant history, so those iterations mix linearly (the reference scheme)
+  // and the history restarts
+  const bool bro = c.mixing == NHF_MIX_BROYDEN && P.have && P.cons_ok;
+  if (c.mixing == NHF_MIX_BROYDEN && !P.cons_ok) {
+    P.bro.valid = 0;
+    P.bro.have_prev = false;
+  }
+  if (bro) {
     for (int q = 0; q < 2; ++q)
       launch_densities<R>(st, src[q], P.N3, P.M[q], P.occ[q], P.n, P.h, 1.0, 1, P.rho_o[q], P.tau_o[q], P.J9_o[q],
                           P.divJ_o[q]);
@@ -1214,9 +1226,10 @@ static int field_species(Problem& P, int q, const FieldParams& F) {
   hipStream_t st = P.W[q].st;
   if (q == 1 && P.cfg.coulomb) {
     // relative residual 1e-13 on the FP64 path (the reference solves to
-    // machine precision), 1e-10 next to FP32 states
+    // machine precision), 1e-8 next to FP32 states (the Coulomb energy then
+    // carries an error of order eV, below what FP32 states resolve)
     int its = 0;
-    CHK(poisson_dev(P, st, P.rho[1], P.UCdir, P.prec == NHF_PREC_FP64 ? 1e-13 : 1e-10, P.pmaxit, &its));
+    CHK(poisson_dev(P, st, P.rho[1], P.UCdir, P.prec == NHF_PREC_FP64 ? 1e-13 : 1e-8, P.pmaxit, &its));
   }
   const double* rho[2] = {P.rho[0], P.rho[1]};
   const double* tau[2] = {P.tau[0], P.tau[1]};
@@ -1439,6 +1452,7 @@ static int scf_dev(Problem& P, std::vector<double> eps[2], nhf_result* res, doub
       double e = constraint_error(k);
       if (e != 0.0) cc = cc && (e < c.constraints_tol);
     }
+    P.cons_ok = cc;
     const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (trace && it < cap) {
       trace[4 * it + 0] = Eint;
@@ -1783,6 +1797,7 @@ static int reset_run_state(Problem& Pr) {
   P->bro.next = 0;
   P->bro.valid = 0;
   P->bro.have_prev = false;
+  P->cons_ok = P->cfg.n_con == 0;
   P->energyDiff = 1.0;
   P->cons.clear();
   for (int i = 0; i < P->cfg.n_con; ++i) {

@@ -120,12 +120,14 @@ def test_lcp_training_and_ablation(ctx):
     # and the SCF needs far fewer iterations than with one Jacobi step
     assert 0 < k_l < 0.5 * k_j
     # the net's correction as the start of a short Jacobi-PCG (4 steps)
-    # beats the same 4 steps started from x (16 vs 21 HF iterations to
-    # 1 keV measured) and reaches the same fixed point
+    # needs no more HF iterations than the same 4 steps started from x
+    # (16-20 vs 20-22 to 1 keV measured over runs: after four PCG steps the
+    # net's head start is worth 0-5 iterations) and reaches the same fixed
+    # point
     r_h, k_h, s_h = run(C.W_LEARNED, 80, pcg=4)
     r_4, k_4, _ = run(C.W_JACOBI_CG, 4)
     print(f"iterations to 1 keV: learned + 4 PCG steps {k_h} (fallbacks {s_h['n']['fallbacks']}), "
           f"4 PCG steps {k_4}")
     for r in (r_h, r_4):
         assert abs(r.terms[C.E_INT] - gold) < 1e-3
-    assert 0 < k_h < k_4
+    assert 0 < k_h <= k_4
