@@ -42,30 +42,31 @@ static std::atomic<int> g_team_threads{0};  // extra members currently running
 
 class Team {
  public:
-  explicit Team(int want) {
-    int extra = std::max(0, want - 1);
+  explicit Team(int want) : want_(std::max(1, want)) {}
+  Team(const Team&) = delete;
+  Team& operator=(const Team&) = delete;
+  int want() const { return want_; }
+  int size() const { return n_; }  // members of the current (or last) run
+  template <class F>
+  void run(F&& f) {
+    // take at most the CPUs not already held by other teams (half the
+    // process's CPUs are left for the solvers' own threads)
+    int extra = want_ - 1;
     if (extra > 0) {
-      // take at most the CPUs not already held by other teams (one per
-      // concurrent solver is left for the callers themselves)
       const int cap = std::max(0, cpus_allowed() / 2 - g_team_threads.load(std::memory_order_relaxed));
       extra = std::min(extra, cap);
       g_team_threads.fetch_add(extra, std::memory_order_relaxed);
     }
     n_ = 1 + extra;
-  }
-  ~Team() {
-    if (n_ > 1) g_team_threads.fetch_sub(n_ - 1, std::memory_order_relaxed);
-  }
-  Team(const Team&) = delete;
-  Team& operator=(const Team&) = delete;
-  int size() const { return n_; }
-  template <class F>
-  void run(F&& f) {
-    std::vector<std::thread> th;
-    th.reserve(n_ - 1);
-    for (int t = 1; t < n_; ++t) th.emplace_back([&f, t] { f(t); });
-    f(0);
-    for (auto& x : th) x.join();
+    cnt_.store(0, std::memory_order_relaxed);
+    {
+      std::vector<std::thread> th;
+      th.reserve(n_ - 1);
+      for (int t = 1; t < n_; ++t) th.emplace_back([&f, t] { f(t); });
+      f(0);
+      for (auto& x : th) x.join();
+    }
+    if (extra > 0) g_team_threads.fetch_sub(extra, std::memory_order_relaxed);
   }
   void barrier() {
     if (n_ == 1) return;
@@ -85,7 +86,7 @@ class Team {
   }
 
  private:
-  int n_ = 1;
+  int want_, n_ = 1;
   std::atomic<int> cnt_{0}, gen_{0};
 };
 
@@ -263,11 +264,11 @@ bool eigh_lower(int k, const cd* Ain, double* w, cd* V, int nvec) {
   std::vector<double> vsr((size_t)k * k, 0.0), vsi((size_t)k * k, 0.0);  // reflector i in column i
   std::vector<double> xr(k), xi(k);
   Team team(team_size(k));
-  const int T = team.size();
   // per-member partial sums of y = A22 v (member 0 accumulates into xr, xi)
-  std::vector<double> pr((size_t)T * k), pi((size_t)T * k);
+  std::vector<double> pr((size_t)team.want() * k), pi((size_t)team.want() * k);
   cd ti_sh = 0.0;  // the current step's tau, published by member 0
   team.run([&](int tid) {
+    const int T = team.size();
     for (int i = 0; i < k - 1; ++i) {
       const int m = k - i - 1, o = i + 1;
       double* vr = &vsr[(size_t)i * k];
@@ -427,35 +428,58 @@ int cholqr_coeffs(int k, const cd* G, double drop_rel, std::vector<int>& keep, s
   double mr = 1.0;
   // R (ke x ke upper, column-major with leading dimension k) grown column by
   // column: R^H R = G[keep, keep]
-  std::vector<cd> R((size_t)k * k, 0.0), rj(k);
+  // Columns go in blocks of CB: the forward substitution of a block's
+  // columns against the columns kept before the block is independent per
+  // column (a team shares it for large bases); the rest of each column's
+  // substitution (against the columns kept inside the block) and the drop
+  // decision follow in column order.  Each column sees the same operations
+  // in the same order as in a column-by-column sweep.
+  constexpr int CB = 32;
+  std::vector<cd> R((size_t)k * k, 0.0), RJ((size_t)CB * k);
   int ke = 0;
-  for (int j = 0; j < k; ++j) {
-    double s = 0.0;
-    for (int i = 0; i < ke; ++i) {
-      const cd* Ri = &R[(size_t)i * k];  // column i of R, rows 0..i
-      cd v = g(keep[i], j);
-      for (int l = 0; l < i; ++l) v -= std::conj(Ri[l]) * rj[l];
-      v /= Ri[i].real();
-      rj[i] = v;
-      s += std::norm(v);
+  Team team(team_size(k));
+  for (int j0 = 0; j0 < k; j0 += CB) {
+    const int nb = std::min(CB, k - j0), ke0 = ke;
+    team.run([&](int tid) {
+      for (int b = tid; b < nb; b += team.size()) {
+        cd* rj = &RJ[(size_t)b * k];
+        for (int i = 0; i < ke0; ++i) {
+          const cd* Ri = &R[(size_t)i * k];  // column i of R, rows 0..i
+          cd v = g(keep[i], j0 + b);
+          for (int l = 0; l < i; ++l) v -= std::conj(Ri[l]) * rj[l];
+          rj[i] = v / Ri[i].real();
+        }
+      }
+    });
+    for (int b = 0; b < nb; ++b) {
+      const int j = j0 + b;
+      cd* rj = &RJ[(size_t)b * k];
+      for (int i = ke0; i < ke; ++i) {
+        const cd* Ri = &R[(size_t)i * k];
+        cd v = g(keep[i], j);
+        for (int l = 0; l < i; ++l) v -= std::conj(Ri[l]) * rj[l];
+        rj[i] = v / Ri[i].real();
+      }
+      double s = 0.0;
+      for (int i = 0; i < ke; ++i) s += std::norm(rj[i]);
+      const double gjj = g(j, j).real();
+      const double dd = gjj - s;
+      if (!(dd > drop_rel * gjj) || !(gjj > 0.0)) continue;
+      mr = std::min(mr, dd / gjj);
+      cd* Rc = &R[(size_t)ke * k];
+      for (int i = 0; i < ke; ++i) Rc[i] = rj[i];
+      Rc[ke] = std::sqrt(dd);
+      keep.push_back(j);
+      ++ke;
     }
-    const double gjj = g(j, j).real();
-    const double dd = gjj - s;
-    if (!(dd > drop_rel * gjj) || !(gjj > 0.0)) continue;
-    mr = std::min(mr, dd / gjj);
-    cd* Rc = &R[(size_t)ke * k];
-    for (int i = 0; i < ke; ++i) Rc[i] = rj[i];
-    Rc[ke] = std::sqrt(dd);
-    keep.push_back(j);
-    ++ke;
   }
   if (min_rel) *min_rel = mr;
   // R^-1 by back substitution, column by column (X upper, ld k)
   std::vector<cd> X((size_t)k * k, 0.0);
-  Team team(team_size(ke));
-  team.run([&](int tid) {
+  Team inv(team_size(ke));
+  inv.run([&](int tid) {
     // columns from the last (the costliest) down, cyclically over the team
-    for (int c = ke - 1 - tid; c >= 0; c -= team.size()) {
+    for (int c = ke - 1 - tid; c >= 0; c -= inv.size()) {
       cd* Xc = &X[(size_t)c * k];
       Xc[c] = 1.0 / R[(size_t)c * k + c].real();
       for (int r = c - 1; r >= 0; --r) {
