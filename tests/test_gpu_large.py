@@ -6,6 +6,7 @@ to run to convergence, through properties that do not depend on size:
   against the oracle's converged energies and single-particle energies;
   the accelerated path (FP32 states, split-BF16 block products, Broyden
   mixing, 8-step Jacobi-PCG) within 1 keV of the oracle.
+* Sn-120, 48^3, 70+56 states, proton BCS: the same two checks.
 * Pb-208, 56^3, 160+110 states, two-basis HFB: the FP64 path's first
   iterations (BCS seed steps, then HFB) against the oracle's E_int
   trace; three seeded guess perturbations run concurrently on the
@@ -73,7 +74,7 @@ def check_fp64(res, g, cfg, tol_e):
     assert abs(np.sqrt(r.r2_n) - g["rms_n"]) < 1e-6 and abs(np.sqrt(r.r2_p) - g["rms_p"]) < 1e-6
 
 
-@pytest.mark.parametrize("name,fixture,tol_e", [("o16", "o16_oracle.json", 1e-5)])
+@pytest.mark.parametrize("name,fixture,tol_e", [("o16", "o16_oracle.json", 1e-5), ("sn120", "sn120_oracle.json", 2e-5)])
 def test_fp64_and_accelerated_match_oracle(ctx, name, fixture, tol_e):
     g = gold(fixture)
     c64 = C.workload(name, precision=C.PREC_FP64)
