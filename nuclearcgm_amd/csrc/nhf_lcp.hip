@@ -424,22 +424,33 @@ __global__ __launch_bounds__(LBS) void k_pair_store(const Sp<R>* __restrict__ T,
 }
 
 
-// ---- BF16 MFMA inference (v_mfma_f32_16x16x16_bf16).  A workgroup owns a
-// brick of 8 x 8 x 4 lattice points; the brick and its one-point halo
-// (10 x 10 x 6 points, zeros outside the box) are staged once in LDS, so
-// every activation is read from HBM/L2 once per brick instead of once per
-// tap.  A wavefront computes tiles of 16 points (8 in x, 2 in y) x 16 output
-// channels; K runs over taps x input channels.  Operand layout of the
-// 16x16x16 MFMA: lane l holds A[l % 16][4 (l / 16) + r], B[4 (l / 16) + r][l % 16]
-// and D[4 (l / 16) + r][l % 16], r = 0..3.  Activations between layers are
-// BF16 (n^3 x 16 per column); weights are per-lane B fragments prepared from
-// the FP32 master copy (k_make_frags).
+// ---- BF16 MFMA inference (v_mfma_f32_16x16x32_bf16).  A workgroup owns an
+// output tile of TX x TY = 14 x 8 lattice points in every plane of a range
+// of zc planes.  The input planes of the tile and its one-point halo (16 x 10
+// points, zeros outside the box) sit in a ring of RING planes in LDS; the
+// next four planes are loaded while the current four are computed (one plane
+// per wavefront).  A wavefront tile is one x row of the 16 input points
+// x0 - 1 .. x0 + 14: the 16 points of each of the nine (dy, dz) input rows
+// are read from LDS once and multiplied by the weights of the three x taps
+// into three accumulators acc_dx (column p = input point p); output point j
+// is acc_0[j] + acc_1[j + 1] + acc_2[j + 2] (DPP row shifts), 14 outputs per
+// 16 lanes.  The weights are the A operand, so the lane of point p holds
+// output channels 4g .. 4g + 3 (g = lane / 16) and stores 8 bytes.  Operand
+// layout of the 16x16x32 MFMA: A[l % 16][8 (l / 16) + r], B[8 (l / 16) + r]
+// [l % 16], D[4 (l / 16) + r][l % 16].  K runs over two input rows x 16
+// channels (layer 0: four rows x 8 channels); row slots past the ninth carry
+// zero weights.  Activations between layers are BF16 (n^3 x 16 per column);
+// the weight fragments are prepared from the FP32 master copy (k_make_frags).
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-constexpr int FRAG_L0 = 0, FRAG_L1 = 14, FRAG_L2 = 41, FRAG_L3 = 68, FRAG_N = 95;
-constexpr int BX = 8, BY = 8, BZ = 4, HX = BX + 2, HY = BY + 2, HZ = BZ + 2, HPTS = HX * HY * HZ;
-constexpr int S16 = 20;  // LDS stride (bf16) of a 16-channel point: conflict-free 8-byte reads
-constexpr int S8 = 12;   // LDS stride of the 8-channel input point
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int TX = 14, TY = 8, PX = TX + 2, PY = TY + 2, PPL = PX * PY;  // halo plane: 160 points
+constexpr int RING = 10;   // LDS planes: the six read by a step + the four loaded for the next
+constexpr int NPRE = 5;    // 16-byte loads per thread for four 16-channel planes
+constexpr int NPRE0 = 3;   // points per thread for four input planes of layer 0
+// fragment sets: layer 0 (dx, group of four rows) 3 x 3; layers 1-3 (dx, row pair) 3 x 5
+constexpr int FR_L0 = 0, FR_L1 = 9, FR_L2 = 24, FR_L3 = 39, FR_N = 54;
 
 __device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest even
   unsigned u = __float_as_uint(x);
@@ -447,171 +458,238 @@ __device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest e
   return (unsigned short)(u >> 16);
 }
 
-// fragments: layer 0 has 14 MFMAs over 2 taps x 8 input channels (residual
-// 0-3, kappa 4, zero 5-7), layers 1-3 one per tap (layer 3 padded to 16
-// output channels with zeros)
-__global__ __launch_bounds__(LBS) void k_make_frags(const float* __restrict__ w, bf16x4* __restrict__ frags) {
+__global__ __launch_bounds__(LBS) void k_make_frags(const float* __restrict__ w, u32x4* __restrict__ frags) {
   const int id = blockIdx.x * LBS + threadIdx.x;
-  if (id >= FRAG_N * 64) return;
-  const int m = id / 64, lane = id % 64, g = lane / 16, col = lane % 16;
-  float v[4];
-  for (int r = 0; r < 4; ++r) {
+  if (id >= FR_N * 64) return;
+  const int f = id / 64, lane = id % 64, g = lane / 16, co = lane % 16;
+  const int layer = f < FR_L1 ? 0 : f < FR_L2 ? 1 : f < FR_L3 ? 2 : 3;
+  const int m = f - (layer == 0 ? FR_L0 : layer == 1 ? FR_L1 : layer == 2 ? FR_L2 : FR_L3);
+  const int grps = layer == 0 ? 3 : 5, dx = m / grps, grp = m % grps;
+  const int CI = layer == 0 ? 5 : 16, CO = layer == 3 ? 4 : 16;
+  unsigned v[8];
+  for (int r = 0; r < 8; ++r) {
+    const int row = layer == 0 ? 4 * grp + g : 2 * grp + g / 2;
+    const int ci = layer == 0 ? r : 8 * (g % 2) + r;
     float x = 0.f;
-    if (m < FRAG_L1) {
-      const int t = 2 * m + g / 2, ci = 4 * (g % 2) + r;
-      if (t < LCP_TAPS && ci < 5) x = w[lcp_layer_offset(0) + (t * 5 + ci) * 16 + col];
-    } else if (m < FRAG_L3) {
-      const int l = m < FRAG_L2 ? 1 : 2, t = m - (l == 1 ? FRAG_L1 : FRAG_L2);
-      x = w[lcp_layer_offset(l) + (t * 16 + 4 * g + r) * 16 + col];
-    } else {
-      const int t = m - FRAG_L3;
-      if (col < 4) x = w[lcp_layer_offset(3) + (t * 16 + 4 * g + r) * 4 + col];
+    if (row < 9 && ci < CI && co < CO) {
+      const int t = (row / 3) * 9 + (row % 3) * 3 + dx;  // (dz, dy, dx) + 1
+      x = w[lcp_layer_offset(layer) + (t * CI + ci) * CO + co];
     }
-    v[r] = x;
+    v[r] = f2bf(x);
   }
-  frags[id] = bf16x4{(short)f2bf(v[0]), (short)f2bf(v[1]), (short)f2bf(v[2]), (short)f2bf(v[3])};
+  frags[id] = u32x4{v[0] | v[1] << 16, v[2] | v[3] << 16, v[4] | v[5] << 16, v[6] | v[7] << 16};
 }
 
-struct Brick {
-  int x0, y0, z0;
+struct Tile {
+  int x0, y0, z0, z1;
 };
-__device__ __forceinline__ Brick brick_of(int n) {
-  const int nbx = (n + BX - 1) / BX, nby = (n + BY - 1) / BY;
-  const int b = blockIdx.x;
-  return Brick{(b % nbx) * BX, ((b / nbx) % nby) * BY, (b / (nbx * nby)) * BZ};
+__device__ __forceinline__ Tile tile_of(int n, int zc) {
+  const int ntx = (n + TX - 1) / TX, nty = (n + TY - 1) / TY, b = blockIdx.x;
+  const int z0 = (b / (ntx * nty)) * zc;
+  return Tile{(b % ntx) * TX, ((b / ntx) % nty) * TY, z0, min(z0 + zc, n)};
 }
-__host__ __device__ inline unsigned bricks_of(int n) {
-  return (unsigned)(((n + BX - 1) / BX) * ((n + BY - 1) / BY) * ((n + BZ - 1) / BZ));
+inline int tile_zc(int n) { return n >= 32 ? 20 : 8; }
+inline unsigned tiles_of(int n, int zc) {
+  return (unsigned)(((n + TX - 1) / TX) * ((n + TY - 1) / TY) * ((n + zc - 1) / zc));
 }
-// halo point h -> lattice coordinates; false outside the box
-__device__ __forceinline__ bool halo_point(const Brick& B, int h, int n, size_t& q) {
-  const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
-  const int x = B.x0 + hx - 1, y = B.y0 + hy - 1, z = B.z0 + hz - 1;
+// ring slot of plane p (p >= -1)
+__device__ __forceinline__ int ring_slot(int p) { return (p + 1) % RING; }
+// halo point h of the planes p0, p0 + 1, ... -> lattice index; false outside the box
+__device__ __forceinline__ bool tile_point(const Tile& T, int h, int p0, int n, size_t& q) {
+  const int x = T.x0 + h % PX - 1, y = T.y0 + (h / PX) % PY - 1, z = p0 + h / PPL;
   if (x < 0 || x >= n || y < 0 || y >= n || z < 0 || z >= n) return false;
   q = (size_t)x + (size_t)n * ((size_t)y + (size_t)n * z);
   return true;
 }
+// lane j <- lane j + k of its row of 16 lanes (0 past the row's end)
+__device__ __forceinline__ float row_shl(float v, int k) {
+  const int i = __float_as_int(v);
+  return __int_as_float(k == 1 ? __builtin_amdgcn_update_dpp(0, i, 0x101, 0xf, 0xf, true)
+                               : __builtin_amdgcn_update_dpp(0, i, 0x102, 0xf, 0xf, true));
+}
+__device__ __forceinline__ void mfma32(f32x4& acc, const u32x4& a, const u32x4& b) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc, 0,
+                                                0, 0);
+}
 
 // layer 0: residual column (scaled by 1/s) and kappa -> 16 channels
 template <typename R>
-__global__ __launch_bounds__(LBS) void k_m_in(const Sp<R>* __restrict__ r, size_t ld, const int* __restrict__ clist,
-                                              const double* __restrict__ rn2, const float* __restrict__ kap,
-                                              const bf16x4* __restrict__ frags, const float* __restrict__ w,
-                                              unsigned short* __restrict__ out, int n, int rs) {
-  __shared__ __align__(16) unsigned short lds[HPTS * S8];
+__global__ __launch_bounds__(LBS) __attribute__((amdgpu_waves_per_eu(3))) void k_m_in(
+    const Sp<R>* __restrict__ r, size_t ld, const int* __restrict__ clist, const double* __restrict__ rn2,
+    const float* __restrict__ kap, const u32x4* __restrict__ frags, const float* __restrict__ w,
+    unsigned short* __restrict__ out, int n, int rs, int zc) {
+  __shared__ __align__(16) unsigned short lds[RING * PPL * 8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
   const size_t N3 = (size_t)n * n * n;
   const int c = clist ? clist[blockIdx.y] : (int)blockIdx.y;
   const double s2 = rn2[(size_t)c * rs] / (4.0 * (double)N3);
   const float inv = s2 > 0.0 ? (float)(1.0 / sqrt(s2)) : 0.f;
   const Sp<R>* __restrict__ src = r + (size_t)c * ld;
-  const Brick Bk = brick_of(n);
-  for (int h = threadIdx.x; h < HPTS; h += LBS) {
-    size_t q;
-    unsigned short v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (halo_point(Bk, h, n, q)) {
-      const Sp<R> x = src[q];
-      v[0] = f2bf((float)x.ur * inv);
-      v[1] = f2bf((float)x.ui * inv);
-      v[2] = f2bf((float)x.dr * inv);
-      v[3] = f2bf((float)x.di * inv);
-      v[4] = f2bf(kap[q]);
-    }
-    bf16x4* d = reinterpret_cast<bf16x4*>(lds + h * S8);
-    d[0] = bf16x4{(short)v[0], (short)v[1], (short)v[2], (short)v[3]};
-    d[1] = bf16x4{(short)v[4], 0, 0, 0};
-  }
-  bf16x4 B[14];
-#pragma unroll
-  for (int m = 0; m < 14; ++m) B[m] = frags[(FRAG_L0 + m) * 64 + lane];
-  const float bs = w[lcp_layer_offset(0) + LCP_TAPS * 5 * 16 + col];
-  __syncthreads();
   unsigned short* __restrict__ dst = out + (size_t)blockIdx.y * N3 * 16;
-  const int xa = col % 8;
-  for (int yp = 0; yp < BY / 2; ++yp) {
-    const int ya = 2 * yp + col / 8, za = wave;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  const Tile T = tile_of(n, zc);
+  Sp<R> sv[NPRE0];
+  float kv[NPRE0];
+  bool in[NPRE0];
+  auto load = [&](int p0, int np) {
 #pragma unroll
-    for (int m = 0; m < 14; ++m) {
-      const int t = 2 * m + g / 2;
-      bf16x4 a = {0, 0, 0, 0};
-      if (t < LCP_TAPS) {
-        const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // halo offsets 0..2
-        const int h = ((za + dz) * HY + (ya + dy)) * HX + (xa + dx);
-        a = *reinterpret_cast<const bf16x4*>(lds + h * S8 + 4 * (g % 2));
+    for (int i = 0; i < NPRE0; ++i) {
+      const int h = threadIdx.x + i * LBS;
+      size_t q = 0;
+      in[i] = h < np * PPL && tile_point(T, h, p0, n, q);
+      if (in[i]) {
+        sv[i] = src[q];
+        kv[i] = kap[q];
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[m], acc, 0, 0, 0);
     }
+  };
+  auto store = [&](int p0, int np) {
 #pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-      const int row = 4 * g + rr;
-      const int x = Bk.x0 + row % 8, y = Bk.y0 + 2 * yp + row / 8, z = Bk.z0 + za;
-      if (x < n && y < n && z < n)
-        dst[((size_t)x + (size_t)n * ((size_t)y + (size_t)n * z)) * 16 + col] = f2bf(leaky(acc[rr] + bs));
+    for (int i = 0; i < NPRE0; ++i) {
+      const int h = threadIdx.x + i * LBS;
+      if (h >= np * PPL) continue;
+      u32x4 v = {0u, 0u, 0u, 0u};
+      if (in[i]) {
+        v[0] = f2bf((float)sv[i].ur * inv) | (unsigned)f2bf((float)sv[i].ui * inv) << 16;
+        v[1] = f2bf((float)sv[i].dr * inv) | (unsigned)f2bf((float)sv[i].di * inv) << 16;
+        v[2] = f2bf(kv[i]);
+      }
+      *reinterpret_cast<u32x4*>(lds + (ring_slot(p0 + h / PPL) * PPL + h % PPL) * 8) = v;
     }
+  };
+  load(T.z0 - 1, 3);
+  store(T.z0 - 1, 3);
+  load(T.z0 + 2, 3);
+  store(T.z0 + 2, 3);
+  u32x4 A[3][3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+    for (int gr = 0; gr < 3; ++gr) A[dx][gr] = frags[(FR_L0 + dx * 3 + gr) * 64 + lane];
+  float bs[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) bs[q] = w[lcp_layer_offset(0) + LCP_TAPS * 5 * 16 + 4 * g + q];
+  __syncthreads();
+  for (int zb = T.z0; zb < T.z1; zb += 4) {
+    const bool more = zb + 4 < T.z1;
+    if (more) load(zb + 5, 4);
+    const int za = zb + wave;
+#pragma unroll 1
+    for (int yr = 0; yr < TY; ++yr) {
+      f32x4 acc[3] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+      for (int gr = 0; gr < 3; ++gr) {
+        const int row = min(4 * gr + g, 8), dy = row % 3, dz = row / 3;
+        const u32x4 b = *reinterpret_cast<const u32x4*>(lds + (ring_slot(za + dz - 1) * PPL + (yr + dy) * PX + col) * 8);
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) mfma32(acc[dx], A[dx][gr], b);
+      }
+      const int x = T.x0 + col, y = T.y0 + yr;
+      float o[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o[q] = acc[0][q] + row_shl(acc[1][q], 1) + row_shl(acc[2][q], 2);
+      if (col < TX && x < n && y < n && za < T.z1) {
+        bf16x4 ob;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ob[q] = (short)f2bf(leaky(o[q] + bs[q]));
+        *reinterpret_cast<bf16x4*>(dst + ((size_t)x + (size_t)n * ((size_t)y + (size_t)n * za)) * 16 + 4 * g) = ob;
+      }
+    }
+    if (more) store(zb + 5, 4);
+    __syncthreads();
   }
 }
 
 // layers 1-2 (16 -> 16, LeakyReLU) and 3 (16 -> 4, scaled into the
 // correction column)
 template <int LAYER, typename R>
-__global__ __launch_bounds__(LBS) void k_m_hid(const unsigned short* __restrict__ in,
-                                               const bf16x4* __restrict__ frags, const float* __restrict__ w,
-                                               unsigned short* __restrict__ out, Sp<R>* __restrict__ z, size_t ld,
-                                               const int* __restrict__ clist, const double* __restrict__ rn2,
-                                               const float* __restrict__ hf, int n, int rs) {
-  constexpr int F0 = LAYER == 1 ? FRAG_L1 : LAYER == 2 ? FRAG_L2 : FRAG_L3;
+__global__ __launch_bounds__(LBS) __attribute__((amdgpu_waves_per_eu(3))) void k_m_hid(
+    const unsigned short* __restrict__ in, const u32x4* __restrict__ frags, const float* __restrict__ w,
+    unsigned short* __restrict__ out, Sp<R>* __restrict__ z, size_t ld, const int* __restrict__ clist,
+    const double* __restrict__ rn2, const float* __restrict__ hf, int n, int rs, int zc) {
+  constexpr int F0 = LAYER == 1 ? FR_L1 : LAYER == 2 ? FR_L2 : FR_L3;
   constexpr int CO = LAYER == 3 ? 4 : 16;
-  __shared__ __align__(16) unsigned short lds[HPTS * S16];
+  __shared__ __align__(16) unsigned short lds[RING * PPL * 16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
   const size_t N3 = (size_t)n * n * n;
   const unsigned short* __restrict__ src = in + (size_t)blockIdx.y * N3 * 16;
-  const Brick Bk = brick_of(n);
+  const Tile T = tile_of(n, zc);
+  uint4 v[NPRE];
   // two 16-byte halves per halo point
-  for (int e = threadIdx.x; e < 2 * HPTS; e += LBS) {
-    const int h = e >> 1, half = e & 1;
-    size_t q;
-    uint4 v = make_uint4(0u, 0u, 0u, 0u);
-    if (halo_point(Bk, h, n, q)) v = *reinterpret_cast<const uint4*>(src + q * 16 + 8 * half);
-    uint2* d = reinterpret_cast<uint2*>(lds + h * S16 + 8 * half);
-    d[0] = make_uint2(v.x, v.y);
-    d[1] = make_uint2(v.z, v.w);
-  }
-  bf16x4 B[LCP_TAPS];
+  auto load = [&](int p0, int np) {
 #pragma unroll
-  for (int t = 0; t < LCP_TAPS; ++t) B[t] = frags[(F0 + t) * 64 + lane];
-  const float bs = col < CO ? w[lcp_layer_offset(LAYER) + LCP_TAPS * 16 * CO + col] : 0.f;
-  __syncthreads();
-  const int xa = col % 8;
+    for (int i = 0; i < NPRE; ++i) {
+      const int e = threadIdx.x + i * LBS;
+      size_t q = 0;
+      v[i] = make_uint4(0u, 0u, 0u, 0u);
+      if ((e >> 1) < np * PPL && tile_point(T, e >> 1, p0, n, q))
+        v[i] = *reinterpret_cast<const uint4*>(src + q * 16 + 8 * (e & 1));
+    }
+  };
+  auto store = [&](int p0, int np) {
+#pragma unroll
+    for (int i = 0; i < NPRE; ++i) {
+      const int e = threadIdx.x + i * LBS, h = e >> 1;
+      if (h < np * PPL) *reinterpret_cast<uint4*>(lds + (ring_slot(p0 + h / PPL) * PPL + h % PPL) * 16 + 8 * (e & 1)) = v[i];
+    }
+  };
+  load(T.z0 - 1, 3);
+  store(T.z0 - 1, 3);
+  load(T.z0 + 2, 3);
+  store(T.z0 + 2, 3);
+  u32x4 A[3][5];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+    for (int pr = 0; pr < 5; ++pr) A[dx][pr] = frags[(F0 + dx * 5 + pr) * 64 + lane];
+  float bs[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) bs[q] = 4 * g + q < CO ? w[lcp_layer_offset(LAYER) + LCP_TAPS * 16 * CO + 4 * g + q] : 0.f;
   float s = 0.f;
   int c = 0;
   if (LAYER == 3) {
     c = clist ? clist[blockIdx.y] : (int)blockIdx.y;
     s = (float)sqrt(rn2[(size_t)c * rs] / (4.0 * (double)N3));
   }
-  for (int yp = 0; yp < BY / 2; ++yp) {
-    const int ya = 2 * yp + col / 8, za = wave;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  __syncthreads();
+  for (int zb = T.z0; zb < T.z1; zb += 4) {
+    const bool more = zb + 4 < T.z1;
+    if (more) load(zb + 5, 4);
+    const int za = zb + wave;
+#pragma unroll 1
+    for (int yr = 0; yr < TY; ++yr) {
+      f32x4 acc[3] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
-    for (int t = 0; t < LCP_TAPS; ++t) {
-      const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
-      const int h = ((za + dz) * HY + (ya + dy)) * HX + (xa + dx);
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(lds + h * S16 + 4 * g);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[t], acc, 0, 0, 0);
-    }
+      for (int pr = 0; pr < 5; ++pr) {
+        const int row = min(2 * pr + (g >> 1), 8), dy = row % 3, dz = row / 3;
+        const u32x4 b = *reinterpret_cast<const u32x4*>(
+            lds + (ring_slot(za + dz - 1) * PPL + (yr + dy) * PX + col) * 16 + 8 * (g & 1));
 #pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-      const int row = 4 * g + rr;
-      const int x = Bk.x0 + row % 8, y = Bk.y0 + 2 * yp + row / 8, zz = Bk.z0 + za;
-      if (x >= n || y >= n || zz >= n) continue;
-      const size_t pt = (size_t)x + (size_t)n * ((size_t)y + (size_t)n * zz);
-      if (LAYER < 3) {
-        out[(size_t)blockIdx.y * N3 * 16 + pt * 16 + col] = f2bf(leaky(acc[rr] + bs));
-      } else if (col < 4) {
-        R* __restrict__ zc = reinterpret_cast<R*>(z + (size_t)c * ld);
-        zc[pt * 4 + col] = (R)((acc[rr] + bs) * s * hf[pt]);
+        for (int dx = 0; dx < 3; ++dx) mfma32(acc[dx], A[dx][pr], b);
+      }
+      const int x = T.x0 + col, y = T.y0 + yr;
+      float o[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o[q] = acc[0][q] + row_shl(acc[1][q], 1) + row_shl(acc[2][q], 2);
+      if (col < TX && x < n && y < n && za < T.z1) {
+        const size_t pt = (size_t)x + (size_t)n * ((size_t)y + (size_t)n * za);
+        if (LAYER < 3) {
+          bf16x4 ob;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) ob[q] = (short)f2bf(leaky(o[q] + bs[q]));
+          *reinterpret_cast<bf16x4*>(out + (size_t)blockIdx.y * N3 * 16 + pt * 16 + 4 * g) = ob;
+        } else if (g == 0) {
+          const float sc = s * hf[pt];
+          Sp<R> zo;
+          zo.ur = (R)((o[0] + bs[0]) * sc);
+          zo.ui = (R)((o[1] + bs[1]) * sc);
+          zo.dr = (R)((o[2] + bs[2]) * sc);
+          zo.di = (R)((o[3] + bs[3]) * sc);
+          z[(size_t)c * ld + pt] = zo;
+        }
       }
     }
+    if (more) store(zb + 5, 4);
+    __syncthreads();
   }
 }
 
@@ -693,14 +771,15 @@ template void launch_lcp_planes<float>(hipStream_t, const float*, size_t, double
 template <typename R>
 void lcp_apply(const Lcp& L, hipStream_t st, const Sp<R>* r, Sp<R>* z, size_t ld, int nb, const int* clist,
                const double* rn2, const float* kap, const float* hf, int n, float* h0, float* h1, int rs) {
-  const dim3 g(bricks_of(n), (unsigned)nb);
-  const bf16x4* fr = reinterpret_cast<const bf16x4*>(L.frags);
+  const int zc = tile_zc(n);
+  const dim3 g(tiles_of(n, zc), (unsigned)nb);
+  const u32x4* fr = reinterpret_cast<const u32x4*>(L.frags);
   unsigned short* a0 = reinterpret_cast<unsigned short*>(h0);
   unsigned short* a1 = reinterpret_cast<unsigned short*>(h1);
-  hipLaunchKernelGGL((k_m_in<R>), g, dim3(LBS), 0, st, r, ld, clist, rn2, kap, fr, L.w, a0, n, rs);
-  hipLaunchKernelGGL((k_m_hid<1, R>), g, dim3(LBS), 0, st, a0, fr, L.w, a1, z, ld, clist, rn2, hf, n, rs);
-  hipLaunchKernelGGL((k_m_hid<2, R>), g, dim3(LBS), 0, st, a1, fr, L.w, a0, z, ld, clist, rn2, hf, n, rs);
-  hipLaunchKernelGGL((k_m_hid<3, R>), g, dim3(LBS), 0, st, a0, fr, L.w, a1, z, ld, clist, rn2, hf, n, rs);
+  hipLaunchKernelGGL((k_m_in<R>), g, dim3(LBS), 0, st, r, ld, clist, rn2, kap, fr, L.w, a0, n, rs, zc);
+  hipLaunchKernelGGL((k_m_hid<1, R>), g, dim3(LBS), 0, st, a0, fr, L.w, a1, z, ld, clist, rn2, hf, n, rs, zc);
+  hipLaunchKernelGGL((k_m_hid<2, R>), g, dim3(LBS), 0, st, a1, fr, L.w, a0, z, ld, clist, rn2, hf, n, rs, zc);
+  hipLaunchKernelGGL((k_m_hid<3, R>), g, dim3(LBS), 0, st, a0, fr, L.w, a1, z, ld, clist, rn2, hf, n, rs, zc);
 }
 // FP32 inference (the training kernels), kept for cross-checks of the BF16 path
 template <typename R>
@@ -718,9 +797,9 @@ void lcp_apply_fp32(const Lcp& L, hipStream_t st, const Sp<R>* r, Sp<R>* z, size
 template void lcp_apply_fp32<float>(const Lcp&, hipStream_t, const Sp<float>*, Sp<float>*, size_t, int, const int*,
                                     const double*, const float*, const float*, int, float*, float*);
 int lcp_make_frags(Lcp& L, hipStream_t st) {
-  if (!L.frags && hipMalloc(&L.frags, sizeof(short) * 4 * 64 * FRAG_N) != hipSuccess) return NHF_ENOMEM;
-  hipLaunchKernelGGL(k_make_frags, dim3((FRAG_N * 64 + LBS - 1) / LBS), dim3(LBS), 0, st, L.w,
-                     reinterpret_cast<bf16x4*>(L.frags));
+  if (!L.frags && hipMalloc(&L.frags, sizeof(u32x4) * 64 * FR_N) != hipSuccess) return NHF_ENOMEM;
+  hipLaunchKernelGGL(k_make_frags, dim3((FR_N * 64 + LBS - 1) / LBS), dim3(LBS), 0, st, L.w,
+                     reinterpret_cast<u32x4*>(L.frags));
   return hipStreamSynchronize(st) == hipSuccess ? NHF_OK : NHF_EHIP;
 }
 template void lcp_apply<double>(const Lcp&, hipStream_t, const Sp<double>*, Sp<double>*, size_t, int, const int*,
