@@ -62,7 +62,8 @@ def time_to_window(trace, e_ref, window=1e-3):
     return float(trace[k, 3]), k + 1
 
 
-WSTEP_PCG = 4  # Jacobi-PCG steps after the learned step (measured best on Ca-40)
+WSTEP_PCG = 2  # Jacobi-PCG steps after the learned step (Ca-40 sweep over 1-6 steps: 2 the fastest
+               # in repeated runs, 78 ms at 18 iterations; 4 moved between 78 and 90 ms)
 
 
 def accelerated_config(workload, wstep=C.W_LEARNED, iters=1, mixing=C.MIX_BROYDEN, steps=80, pcg=WSTEP_PCG):

@@ -655,8 +655,15 @@ __global__ __launch_bounds__(LBS) __attribute__((amdgpu_waves_per_eu(3))) void k
     const bool more = zb + 4 < T.z1;
     if (more) load(zb + 5, 4);
     const int za = zb + wave;
+    // layer 3: the output scale h^2 / f of the tile's points, loaded one tile ahead
+    auto hf_row = [&](int yr) {
+      const int x = T.x0 + col, y = T.y0 + yr;
+      return col < TX && x < n && y < n && za < T.z1 ? hf[(size_t)x + (size_t)n * ((size_t)y + (size_t)n * za)] : 0.f;
+    };
+    float hcur = LAYER == 3 ? hf_row(0) : 0.f;
 #pragma unroll 1
     for (int yr = 0; yr < TY; ++yr) {
+      const float hnext = LAYER == 3 && yr + 1 < TY ? hf_row(yr + 1) : 0.f;
       f32x4 acc[3] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
       for (int pr = 0; pr < 5; ++pr) {
@@ -678,7 +685,7 @@ __global__ __launch_bounds__(LBS) __attribute__((amdgpu_waves_per_eu(3))) void k
           for (int q = 0; q < 4; ++q) ob[q] = (short)f2bf(leaky(o[q] + bs[q]));
           *reinterpret_cast<bf16x4*>(out + (size_t)blockIdx.y * N3 * 16 + pt * 16 + 4 * g) = ob;
         } else if (g == 0) {
-          const float sc = s * hf[pt];
+          const float sc = s * hcur;
           Sp<R> zo;
           zo.ur = (R)((o[0] + bs[0]) * sc);
           zo.ui = (R)((o[1] + bs[1]) * sc);
@@ -687,6 +694,7 @@ __global__ __launch_bounds__(LBS) __attribute__((amdgpu_waves_per_eu(3))) void k
           z[(size_t)c * ld + pt] = zo;
         }
       }
+      hcur = hnext;
     }
     if (more) store(zb + 5, 4);
     __syncthreads();
