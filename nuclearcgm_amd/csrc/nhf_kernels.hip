@@ -920,7 +920,7 @@ constexpr int GSU = 5;   // LDS units per column and split level (4 + 1 pad)
 constexpr int GUNITS = 2 * 3 * GS * GSU;
 constexpr int GWAVES = 12;
 constexpr int GBS = 64 * GWAVES;
-constexpr int GPF = 4;      // slabs of operands in flight per thread
+constexpr int GPF = 2;      // slabs of operands in flight per thread (4 spills registers at 3 waves/SIMD)
 constexpr int GSLABS = 32;  // slabs (8 points) per workgroup
 constexpr int GUNROLL = 8;  // slabs per loop iteration (code size vs. waits at the loop head)
 constexpr int GCHUNK = 8 * GSLABS;
