@@ -319,6 +319,15 @@ int nhf_phase_times(void *ctx, int pid, int reset, double *out);
 int nhf_block_gram(void *ctx, size_t npts, int ka, const double *A, int kb, const double *B, int prec, int lower,
                    double *G);
 
+/* The Ritz rotation of the GCG (X_new_full = V_eff * C, solver.cpp:777) on
+ * host data: Y = V C for a complex spinor block V of npts points (2 npts
+ * complex rows, k columns) and a complex k x ncols coefficient matrix C,
+ * with the device kernel of precision prec (NHF_PREC_MIXED: V is rounded
+ * to FP32 and Y comes from the FP32-state rotation kernel).  V, C, Y
+ * column-major; Y: 2 npts x ncols. */
+int nhf_block_rotate(void *ctx, size_t npts, int k, const double *V, int ncols, const double *C, int prec,
+                     double *Y);
+
 /* Harmonic-oscillator spinor guess |n l j m_j> with oscillator length b
  * (fm), Euclidean-normalised columns (guess.cpp:10-248 with
  * useSpinOrbit = true).  Host-side; out: 2n^3 x nev complex. */

@@ -1400,16 +1400,197 @@ __global__ __launch_bounds__(BS) void k_rotate_mfma(const Sp<float>* __restrict_
   }
 }
 
-size_t rotate_frag_count(int k, int ncols) {
-  return (size_t)((k + 7) / 8) * ((ncols + 15) / 16) * 2 * 64;
+// The rotation of FP32 states with split-BF16 operands on
+// v_mfma_f32_16x16x32_bf16, the block products' scheme: every FP32 value of
+// V and every coefficient is split exactly into three BF16 parts and the six
+// leading products hh + hm + mh + hl + lh + mm are accumulated in FP32 (the
+// dropped ones stay below 2^-24 relative).  Real form as in k_rotate_mfma,
+// with K running over 16 states x (re, im) per MFMA; lane L holds
+// A[L % 16][8 (L / 16) + r], B[8 (L / 16) + r][L % 16] and
+// D[4 (L / 16) + q][L % 16].  A workgroup (4 waves) owns 64 points (128 rows
+// j = 2p + s) and RCT column tiles of 16.  Per chunk of 16 states the V tile
+// sits in LDS as FP32 rows j of 16 states x (re, im) (double-buffered; a row
+// pitch of 36 floats makes the 16-byte reads of 16 consecutive rows
+// conflict-free); each row tile is read by one wavefront only, which splits
+// its operands after the read.  The coefficient fragments (chunk, column
+// tile, re / im form, level) come from k_rot_frags16.
+constexpr int RB_PITCH = 36;  // floats per LDS row (32 + 4 pad)
+
+__global__ void k_rot_frags16(const double2* __restrict__ C, int k, int ncols, int nt, int nu,
+                              gu32x4* __restrict__ F) {
+  const int e = blockIdx.x * BS + threadIdx.x;  // (u, cb, lane)
+  if (e >= nu * nt * 64) return;
+  const int lane = e & 63, cb = (e >> 6) % nt, u = (e >> 6) / nt;
+  const int c = 16 * cb + (lane & 15), g = lane >> 4;
+  // K = 8 g + r: state 16 u + 4 g + r / 2, part r % 2
+  float are[8], aim[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int l = 16 * u + 4 * g + r / 2;
+    double2 v = make_double2(0.0, 0.0);
+    if (l < k && c < ncols) v = C[(size_t)c * k + l];
+    are[r] = (r & 1) ? (float)-v.y : (float)v.x;
+    aim[r] = (r & 1) ? (float)v.x : (float)v.y;
+  }
+  gu32x4 hr, mr, lr, hi, mi, li;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    unsigned h, m, l;
+    split2(are[2 * i], are[2 * i + 1], h, m, l);
+    hr[i] = h;
+    mr[i] = m;
+    lr[i] = l;
+    split2(aim[2 * i], aim[2 * i + 1], h, m, l);
+    hi[i] = h;
+    mi[i] = m;
+    li[i] = l;
+  }
+  gu32x4* f = F + (((size_t)u * nt + cb) * 6) * 64 + lane;
+  f[0] = hr;
+  f[64] = mr;
+  f[128] = lr;
+  f[192] = hi;
+  f[256] = mi;
+  f[320] = li;
 }
 
-// NHF_ROT=valu: the VALU rotation kernel for FP32 states too
+template <int RCT>
+__global__ __launch_bounds__(BS) void k_rotate_bf(const Sp<float>* __restrict__ V, size_t ldv, int k,
+                                                  const gu32x4* __restrict__ F, int nt, int ncols,
+                                                  Sp<float>* __restrict__ Y, size_t ldy, size_t N3, int accumulate) {
+  __shared__ __align__(16) float Bl[2][128 * RB_PITCH];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, jl = lane & 15;
+  const size_t p0 = (size_t)blockIdx.x * 64;
+  const int cb0 = blockIdx.y * RCT;
+  const int nu = (k + 15) / 16;
+  // staging: thread (point sp, group sg) loads states 16 u + 4 sg .. + 3 at
+  // point p0 + sp (a wave: one state group, 64 consecutive points)
+  const int sp = tid & 63, sg = tid >> 6;
+  auto load = [&](int u, float4 (&r)[4]) {
+    const size_t p = p0 + sp;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int l = 16 * u + 4 * sg + i;
+      r[i] = (p < N3 && l < k) ? *reinterpret_cast<const float4*>(V + (size_t)l * ldv + p)
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  // row 2 sp: (up re, up im) of the four states, row 2 sp + 1: (down re, down im)
+  auto store = [&](int buf, const float4 (&r)[4]) {
+    float* b = &Bl[buf][(2 * sp) * RB_PITCH + 8 * sg];
+    *reinterpret_cast<float4*>(b) = make_float4(r[0].x, r[0].y, r[1].x, r[1].y);
+    *reinterpret_cast<float4*>(b + 4) = make_float4(r[2].x, r[2].y, r[3].x, r[3].y);
+    *reinterpret_cast<float4*>(b + RB_PITCH) = make_float4(r[0].z, r[0].w, r[1].z, r[1].w);
+    *reinterpret_cast<float4*>(b + RB_PITCH + 4) = make_float4(r[2].z, r[2].w, r[3].z, r[3].w);
+  };
+  gf32x4 dre[2][RCT], dim[2][RCT];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < RCT; ++b) {
+      dre[a][b] = gf32x4{0.f, 0.f, 0.f, 0.f};
+      dim[a][b] = gf32x4{0.f, 0.f, 0.f, 0.f};
+    }
+  float4 r[4];
+  load(0, r);
+  store(0, r);
+  __syncthreads();
+  for (int u = 0; u < nu; ++u) {
+    const int cur = u & 1;
+    if (u + 1 < nu) load(u + 1, r);
+    gu32x4 bv[2][3];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const float* row = &Bl[cur][((wave * 2 + a) * 16 + jl) * RB_PITCH + 8 * g];
+      const float4 x0 = *reinterpret_cast<const float4*>(row), x1 = *reinterpret_cast<const float4*>(row + 4);
+      const float xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        unsigned h, m, l;
+        split2(xv[2 * i], xv[2 * i + 1], h, m, l);
+        bv[a][0][i] = h;
+        bv[a][1][i] = m;
+        bv[a][2][i] = l;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < RCT; ++b) {
+      const int cb = cb0 + b;
+      if (cb >= nt) continue;
+      const gu32x4* f = F + (((size_t)u * nt + cb) * 6) * 64 + lane;
+      gu32x4 are[3], aim[3];
+#pragma unroll
+      for (int v = 0; v < 3; ++v) {
+        are[v] = f[64 * v];
+        aim[v] = f[64 * (3 + v)];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        // smallest terms first
+        mfma32(dre[a][b], are[1], bv[a][1]);
+        mfma32(dre[a][b], are[2], bv[a][0]);
+        mfma32(dre[a][b], are[0], bv[a][2]);
+        mfma32(dre[a][b], are[1], bv[a][0]);
+        mfma32(dre[a][b], are[0], bv[a][1]);
+        mfma32(dre[a][b], are[0], bv[a][0]);
+        mfma32(dim[a][b], aim[1], bv[a][1]);
+        mfma32(dim[a][b], aim[2], bv[a][0]);
+        mfma32(dim[a][b], aim[0], bv[a][2]);
+        mfma32(dim[a][b], aim[1], bv[a][0]);
+        mfma32(dim[a][b], aim[0], bv[a][1]);
+        mfma32(dim[a][b], aim[0], bv[a][0]);
+      }
+    }
+    // the other image was last read in the previous chunk, and every thread
+    // passed the barrier that ended it
+    if (u + 1 < nu) store(cur ^ 1, r);
+    __syncthreads();
+  }
+  // lane holds D[c = 16 cb + 4 g + q][j = 16 (2 wave + a) + jl]
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int j = (wave * 2 + a) * 16 + jl;
+    const size_t p = p0 + (j >> 1);
+    if (p >= N3) continue;
+    const int s = j & 1;
+#pragma unroll
+    for (int b = 0; b < RCT; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = 16 * (cb0 + b) + 4 * g + q;
+        if (c < ncols) {
+          float2* yp = reinterpret_cast<float2*>(reinterpret_cast<float*>(Y + (size_t)c * ldy + p) + 2 * s);
+          float2 o = make_float2(dre[a][b][q], dim[a][b][q]);
+          if (accumulate) {
+            const float2 old = *yp;
+            o.x += old.x;
+            o.y += old.y;
+          }
+          *yp = o;
+        }
+      }
+  }
+}
+
+// float4 units of the coefficient fragments of either matrix-core kernel
+size_t rotate_frag_count(int k, int ncols) {
+  const size_t nt = (size_t)((ncols + 15) / 16);
+  return std::max((size_t)((k + 7) / 8) * nt * 2 * 64, (size_t)((k + 15) / 16) * nt * 6 * 64);
+}
+
+// NHF_ROT=valu: the VALU rotation kernel for FP32 states too; NHF_ROT=f32:
+// the FP32 matrix-core kernel instead of the split-BF16 one
+static bool rot_env(const char* what) {
+  const char* e = std::getenv("NHF_ROT");
+  return e && std::strcmp(e, what) == 0;
+}
 static bool rot_valu() {
-  static bool v = [] {
-    const char* e = std::getenv("NHF_ROT");
-    return e && std::strcmp(e, "valu") == 0;
-  }();
+  static bool v = rot_env("valu");
+  return v;
+}
+static bool rot_f32() {
+  static bool v = rot_env("f32");
   return v;
 }
 
@@ -1417,6 +1598,22 @@ template <typename R>
 void launch_rotate(hipStream_t st, const Sp<R>* V, size_t ldv, int k, const double2* C, int ncols, Sp<R>* Y,
                    size_t ldy, size_t N3, const Sp<R>* AV, const double* w, double* rn2, int accumulate,
                    float4* frag) {
+  if (std::is_same<R, float>::value && !AV && frag && !block_fp64() && !rot_valu() && !rot_f32()) {
+    const int nt = (ncols + 15) / 16, nu = (k + 15) / 16;
+    const unsigned nfe = (unsigned)(nu * nt * 64);
+    gu32x4* fb = reinterpret_cast<gu32x4*>(frag);
+    hipLaunchKernelGGL(k_rot_frags16, dim3((nfe + BS - 1) / BS), dim3(BS), 0, st, C, k, ncols, nt, nu, fb);
+    const unsigned gx = (unsigned)((N3 + 63) / 64);
+    const Sp<float>* Vf = reinterpret_cast<const Sp<float>*>(V);
+    Sp<float>* Yf = reinterpret_cast<Sp<float>*>(Y);
+    if (nt > 4)
+      hipLaunchKernelGGL((k_rotate_bf<3>), dim3(gx, (unsigned)((nt + 2) / 3)), dim3(BS), 0, st, Vf, ldv, k, fb, nt,
+                         ncols, Yf, ldy, N3, accumulate);
+    else
+      hipLaunchKernelGGL((k_rotate_bf<2>), dim3(gx, (unsigned)((nt + 1) / 2)), dim3(BS), 0, st, Vf, ldv, k, fb, nt,
+                         ncols, Yf, ldy, N3, accumulate);
+    return;
+  }
   if (std::is_same<R, float>::value && !AV && frag && !block_fp64() && !rot_valu()) {
     const int nt = (ncols + 15) / 16, nu = (k + 7) / 8;
     const unsigned nfe = (unsigned)(nu * nt * 64);

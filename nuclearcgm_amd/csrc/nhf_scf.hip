@@ -2147,6 +2147,44 @@ int nhf_block_gram(void* ctx, size_t npts, int ka, const double* A, int kb, cons
   return rc;
 }
 
+int nhf_block_rotate(void* ctx, size_t npts, int k, const double* V, int ncols, const double* Cm, int prec,
+                     double* Y) {
+  if (!ctx || !V || !Cm || !Y || npts < 1 || k < 1 || ncols < 1 ||
+      (prec != NHF_PREC_FP64 && prec != NHF_PREC_MIXED))
+    return NHF_EINVAL;
+  Ctx* c = (Ctx*)ctx;
+  HIPCHK(hipSetDevice(c->device));
+  const bool f64 = prec == NHF_PREC_FP64;
+  const size_t es = f64 ? sizeof(Sp<double>) : sizeof(Sp<float>);
+  void *dv = nullptr, *dy = nullptr, *dc = nullptr, *df = nullptr, *tmp = nullptr;
+  int rc = NHF_OK;
+  if (hipMalloc(&dv, es * npts * k) != hipSuccess || hipMalloc(&dy, es * npts * ncols) != hipSuccess ||
+      hipMalloc(&dc, sizeof(double2) * (size_t)k * ncols) != hipSuccess ||
+      hipMalloc(&df, sizeof(float4) * rotate_frag_count(k, ncols)) != hipSuccess ||
+      hipMalloc(&tmp, sizeof(Sp<double>) * npts * std::max(k, ncols)) != hipSuccess)
+    rc = NHF_ENOMEM;
+  if (rc == NHF_OK) {
+    hipError_t e = hipMemcpy(dc, Cm, sizeof(double2) * (size_t)k * ncols, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(f64 ? dv : tmp, V, sizeof(Sp<double>) * npts * k, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !f64) launch_convert<double, float>(c->st, (const Sp<double>*)tmp, (Sp<float>*)dv, npts * k);
+    if (e == hipSuccess) {
+      if (f64)
+        launch_rotate<double>(c->st, (const Sp<double>*)dv, npts, k, (const double2*)dc, ncols, (Sp<double>*)dy, npts,
+                              npts, nullptr, nullptr, nullptr, 0, (float4*)df);
+      else
+        launch_rotate<float>(c->st, (const Sp<float>*)dv, npts, k, (const double2*)dc, ncols, (Sp<float>*)dy, npts,
+                             npts, nullptr, nullptr, nullptr, 0, (float4*)df);
+      if (!f64) launch_convert<float, double>(c->st, (const Sp<float>*)dy, (Sp<double>*)tmp, npts * ncols);
+      e = hipStreamSynchronize(c->st);
+    }
+    if (e == hipSuccess) e = hipMemcpy(Y, f64 ? dy : tmp, sizeof(Sp<double>) * npts * ncols, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = NHF_EHIP;
+  }
+  for (void* q : {dv, dy, dc, df, tmp})
+    if (q) (void)hipFree(q);
+  return rc;
+}
+
 int nhf_lcp_apply(void* ctx, int n, int m, const float* r, const float* kap, const float* hf, float* z, int mode) {
   if (!ctx || !r || !kap || !hf || !z || n < 3 || m < 1 || (mode != 0 && mode != 1)) return NHF_EINVAL;
   Ctx* c = (Ctx*)ctx;

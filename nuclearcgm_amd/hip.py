@@ -59,6 +59,8 @@ def lib():
         L.nhf_eigh_lower.argtypes = [ctypes.c_int, _D, _D, _D]
         L.nhf_block_gram.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, _D, ctypes.c_int, _D,
                                      ctypes.c_int, ctypes.c_int, _D]
+        L.nhf_block_rotate.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, _D, ctypes.c_int, _D,
+                                       ctypes.c_int, _D]
         L.nhf_cholqr_coeffs.argtypes = [ctypes.c_int, _D, ctypes.c_double, ctypes.POINTER(ctypes.c_int), _D]
         L.nhf_edf_from_skyrme.argtypes = [_D, _D]
         L.nhf_edf_from_skyrme.restype = None
@@ -213,6 +215,15 @@ class Context:
         _chk(lib().nhf_block_gram(self.h, A.shape[0] // 2, A.shape[1], _dp(A), Bm.shape[1], _dp(Bm), prec,
                                   int(lower), _dp(G)))
         return G
+
+    def block_rotate(self, V, Cm, prec=PREC_FP64):
+        """Y = V C with the library's device rotation (V: complex (2 npts, k)
+        column block, C: complex (k, ncols))."""
+        V = _cols(V)
+        Cm = _cols(Cm)
+        Y = np.zeros((V.shape[0], Cm.shape[1]), dtype=np.complex128, order="F")
+        _chk(lib().nhf_block_rotate(self.h, V.shape[0] // 2, V.shape[1], _dp(V), Cm.shape[1], _dp(Cm), prec, _dp(Y)))
+        return Y
 
     # ---- learned convolutional preconditioner (one per context)
     def lcp_init(self, seed):

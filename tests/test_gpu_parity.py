@@ -103,6 +103,31 @@ def test_scf_o16_matches_oracle(H):
     assert abs(r_or[4].r2_p - r_gp[4].r2_p) < 1e-5
 
 
+@pytest.mark.parametrize("npts,k,ncols", [(1000, 7, 5), (27000, 40, 33), (64000, 96, 32), (64000, 96, 96),
+                                          (20001, 150, 130)])
+def test_block_rotate(H, npts, k, ncols):
+    """The Ritz rotation Y = V C: FP64 kernel to rounding; the FP32-state
+    kernel (split-BF16 MFMA: exact three-way split of V and of C rounded to
+    FP32, six leading products, FP32 accumulation) within 1e-6 of max |Y|
+    against the FP64 product of the FP32-rounded inputs.  7 and 150 states
+    leave partial 16-state chunks, 33 and 130 columns partial column tiles,
+    1000 and 20001 points a partial 64-point workgroup; 96 and 130 columns
+    take the three-tile variant."""
+    _, ctx = H
+    rng = np.random.default_rng(npts + k)
+    V = rng.standard_normal((2 * npts, k)) + 1j * rng.standard_normal((2 * npts, k))
+    V /= np.linalg.norm(V, axis=0)
+    Cm = rng.standard_normal((k, ncols)) + 1j * rng.standard_normal((k, ncols))
+    Y0 = V @ Cm
+    assert np.abs(ctx.block_rotate(V, Cm, C.PREC_FP64) - Y0).max() < 1e-13 * np.abs(Y0).max()
+    V32 = V.astype(np.complex64).astype(np.complex128)
+    C32 = Cm.astype(np.complex64).astype(np.complex128)
+    Y32 = V32 @ C32
+    err = np.abs(ctx.block_rotate(V, Cm, C.PREC_MIXED) - Y32).max() / np.abs(Y32).max()
+    print(f"rotation {npts} x {k} -> {ncols}: max |dY| / max |Y| = {err:.2e}")
+    assert err < 1e-6
+
+
 @pytest.mark.parametrize("npts,ka,kb", [(1000, 5, 7), (27000, 40, 33), (64000, 96, 96), (20001, 150, 130)])
 def test_block_gram(H, npts, ka, kb):
     """The GCG block product A^H B: FP64 kernel to rounding; split-BF16 MFMA
