@@ -480,12 +480,13 @@ __global__ __launch_bounds__(BS) void k_cg_init(const Sp<R>* __restrict__ X, con
                                                 size_t ld, double* __restrict__ s0, double* __restrict__ rhs2,
                                                 int from_w) {
   const int c = blockIdx.y;
-  const size_t p = (size_t)blockIdx.x * BS + threadIdx.x;
   double acc[3] = {0.0, 0.0, 0.0};  // |b|^2, |r|^2, r^H z
-  if (p < N3) {
+  const R ls = (R)lam_shift[c];
+  // a capped grid, several points per thread: fewer block reductions and atomics
+#pragma unroll 2
+  for (size_t p = (size_t)blockIdx.x * BS + threadIdx.x; p < N3; p += (size_t)gridDim.x * BS) {
     const size_t o = (size_t)c * ld + p;
     const Sp<R> x = X[o], t = T[o];
-    const R ls = (R)lam_shift[c];
     if (!from_w) W[o] = x;  // else W holds the start vector and T = (H + shift) W
     Sp<R> b{x.ur * ls, x.ui * ls, x.dr * ls, x.di * ls};
     Sp<R> r{b.ur - t.ur, b.ui - t.ui, b.dr - t.dr, b.di - t.di};
@@ -494,9 +495,9 @@ __global__ __launch_bounds__(BS) void k_cg_init(const Sp<R>* __restrict__ X, con
     P[o] = Sp<R>{r.ur * dg, r.ui * dg, r.dr * dg, r.di * dg};
     double bb = (double)b.ur * b.ur + (double)b.ui * b.ui + (double)b.dr * b.dr + (double)b.di * b.di;
     double rr = (double)r.ur * r.ur + (double)r.ui * r.ui + (double)r.dr * r.dr + (double)r.di * r.di;
-    acc[0] = bb;
-    acc[1] = rr;
-    acc[2] = rr * (double)dg;
+    acc[0] += bb;
+    acc[1] += rr;
+    acc[2] += rr * (double)dg;
   }
   // reduce the three values into rhs2[c], s0[c][2], s0[c][3]
   double v1[1] = {acc[0]};
@@ -510,7 +511,7 @@ template <typename R>
 void launch_cg_init(hipStream_t st, const Sp<R>* X, const double* lam_shift, Sp<R>* W, Sp<R>* Rr, Sp<R>* P,
                     const Sp<R>* T, const R* invd, size_t N3, size_t ld, int ncols, double* s0, double* rhs2,
                     int from_w) {
-  dim3 grid((unsigned)((N3 + BS - 1) / BS), (unsigned)ncols);
+  dim3 grid((unsigned)std::min<size_t>((N3 + BS - 1) / BS, 64), (unsigned)ncols);
   hipLaunchKernelGGL((k_cg_init<R>), grid, dim3(BS), 0, st, X, lam_shift, W, Rr, P, T, invd, N3, ld, s0, rhs2,
                      from_w);
 }

@@ -56,10 +56,10 @@ def accelerated(c):
     return c
 
 
-def check_fp64(res, g, cfg, tol_e):
+def check_fp64(res, g, cfg, tol_e, tol_occ):
     """Converged FP64 run against the oracle's: energies, radii, and the
     single-particle energies -- the occupied ones (the lowest N and Z) to
-    1e-5 MeV, the whole block to 1e-3 MeV (empty levels do not enter E_tot,
+    tol_occ, the whole block to 1e-3 MeV (empty levels do not enter E_tot,
     so they are still settling when the energy criterion stops a run; in a
     near-degenerate empty shell they differ by ~1e-4 MeV between runs)."""
     r, en, ep = res[4], res[2], res[3]
@@ -70,17 +70,21 @@ def check_fp64(res, g, cfg, tol_e):
     for eps, ref, nocc in ((en, g["eps_n"], cfg.A - cfg.Z), (ep, g["eps_p"], cfg.Z)):
         occ, allst = bench.spe_errors(eps, ref, nocc)
         print(f"  max |d eps| occupied {occ:.2e}, all {allst:.2e} MeV")
-        assert occ < 1e-5 and allst < 1e-3
+        assert occ < tol_occ and allst < 1e-3
     assert abs(np.sqrt(r.r2_n) - g["rms_n"]) < 1e-6 and abs(np.sqrt(r.r2_p) - g["rms_p"]) < 1e-6
 
 
-@pytest.mark.parametrize("name,fixture,tol_e", [("o16", "o16_oracle.json", 1e-5), ("sn120", "sn120_oracle.json", 2e-5)])
-def test_fp64_and_accelerated_match_oracle(ctx, name, fixture, tol_e):
+# occupied single-particle energies at the |dE| < 1e-6 MeV stop: O-16 1.2e-6
+# MeV; Sn-120 3.9e-6 and 1.8e-5 MeV in two runs whose device FP64 sums ran in
+# different orders (the runs stopped after 65 and 66 iterations)
+@pytest.mark.parametrize("name,fixture,tol_e,tol_occ", [("o16", "o16_oracle.json", 1e-5, 1e-5),
+                                                        ("sn120", "sn120_oracle.json", 2e-5, 5e-5)])
+def test_fp64_and_accelerated_match_oracle(ctx, name, fixture, tol_e, tol_occ):
     g = gold(fixture)
     c64 = C.workload(name, precision=C.PREC_FP64)
     c64.max_iter = 600
     guess = bench.make_guess(c64)
-    check_fp64(ctx.problem(c64).scf_run(*guess), g, c64, tol_e)
+    check_fp64(ctx.problem(c64).scf_run(*guess), g, c64, tol_e, tol_occ)
     cm = accelerated(C.workload(name, precision=C.PREC_MIXED))
     cm.max_iter = 400
     r = ctx.problem(cm).scf_run(*guess)
