@@ -42,6 +42,7 @@ def main(argv=None):
     ap.add_argument("--which", default="o16,curve,sn120,pb208")
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--wstep-iters", type=int, default=1)
+    ap.add_argument("--wstep-pcg", type=int, default=None, help="PCG steps after the learned step (configs)")
     ap.add_argument("--no-train", action="store_true")
     ap.add_argument("--mib", type=int, default=512)
     ap.add_argument("--cols", type=int, default=32)
@@ -63,7 +64,7 @@ def main(argv=None):
 
     ctx = hip.Context(0)
     if a.what == "configs":
-        measure_configs(ctx, a.which.split(","))
+        measure_configs(ctx, a.which.split(","), a.wstep_pcg)
         return
     if a.what == "stencil":
         cfg = C.workload("ca40", precision=C.PREC_MIXED)
@@ -122,7 +123,7 @@ def _errors(bench, C, r, ref, cfg):
             "d_beta2": sm["beta2"] - so["beta2"]}
 
 
-def measure_configs(ctx, which):
+def measure_configs(ctx, which, pcg=None):
     import json
     import time
 
@@ -136,7 +137,7 @@ def measure_configs(ctx, which):
 
     def accel(c):
         c.mixing = C.MIX_BROYDEN
-        c.wstep, c.wstep_iters, c.wstep_pcg = C.W_LEARNED, 1, bench.WSTEP_PCG
+        c.wstep, c.wstep_iters, c.wstep_pcg = C.W_LEARNED, 1, bench.WSTEP_PCG if pcg is None else pcg
         return c
 
     for w in which:
