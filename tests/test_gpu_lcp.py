@@ -48,7 +48,7 @@ def test_lcp_kernels_match_oracle(ctx, n, batch):
 @pytest.mark.parametrize("n", [6, 17, 40])
 def test_lcp_bf16_inference_matches_fp32(ctx, n):
     """BF16 activations and weights with FP32 accumulation: 0.5 % relative
-    (n = 6, 17 leave partial 8 x 8 x 4 bricks)."""
+    (n = 6, 17 leave partial 14 x 8 output tiles and z ranges)."""
     ctx.lcp_init(77)
     rng = np.random.default_rng(n)
     r = rng.standard_normal((3, n ** 3, 4)).astype(np.float32)
