@@ -947,16 +947,15 @@ __global__ __launch_bounds__(LBS) void k_lw_init(const Sp<R>* __restrict__ X, co
                                                  Sp<R>* __restrict__ Rr, size_t N3, size_t ld,
                                                  double* __restrict__ sc) {
   const int c = blockIdx.y;
-  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
+  const R l = (R)ls[c];
   double v[1] = {0.0};
-  if (p < N3) {
+  for (size_t p = (size_t)blockIdx.x * LBS + threadIdx.x; p < N3; p += (size_t)gridDim.x * LBS) {
     const size_t o = (size_t)c * ld + p;
     const Sp<R> x = X[o], t = T[o];
-    const R l = (R)ls[c];
     const Sp<R> r{t.ur - l * x.ur, t.ui - l * x.ui, t.dr - l * x.dr, t.di - l * x.di};
     W[o] = x;
     Rr[o] = r;
-    v[0] = (double)r.ur * r.ur + (double)r.ui * r.ui + (double)r.dr * r.dr + (double)r.di * r.di;
+    v[0] += (double)r.ur * r.ur + (double)r.ui * r.ui + (double)r.dr * r.dr + (double)r.di * r.di;
   }
   block_sum_add<1>(v, sc + (size_t)c * LW_NS);
 }
@@ -964,14 +963,13 @@ template <typename R>
 __global__ __launch_bounds__(LBS) void k_lw_dots(const Sp<R>* __restrict__ Q, const Sp<R>* __restrict__ Rr, size_t N3,
                                                  size_t ld, double* __restrict__ sc) {
   const int c = blockIdx.y;
-  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
-  if (p < N3) {
+  for (size_t p = (size_t)blockIdx.x * LBS + threadIdx.x; p < N3; p += (size_t)gridDim.x * LBS) {
     const size_t o = (size_t)c * ld + p;
     const Sp<R> q = Q[o], r = Rr[o];
-    v[0] = (double)q.ur * r.ur + (double)q.ui * r.ui + (double)q.dr * r.dr + (double)q.di * r.di;
-    v[1] = (double)q.ur * r.ui - (double)q.ui * r.ur + (double)q.dr * r.di - (double)q.di * r.dr;
-    v[2] = (double)q.ur * q.ur + (double)q.ui * q.ui + (double)q.dr * q.dr + (double)q.di * q.di;
+    v[0] += (double)q.ur * r.ur + (double)q.ui * r.ui + (double)q.dr * r.dr + (double)q.di * r.di;
+    v[1] += (double)q.ur * r.ui - (double)q.ui * r.ur + (double)q.dr * r.di - (double)q.di * r.dr;
+    v[2] += (double)q.ur * q.ur + (double)q.ui * q.ui + (double)q.dr * q.dr + (double)q.di * q.di;
   }
   block_sum_add<3>(v, sc + (size_t)c * LW_NS + 1);
 }
@@ -981,16 +979,15 @@ __global__ __launch_bounds__(LBS) void k_lw_update(const Sp<R>* __restrict__ Z, 
                                                    size_t ld, const double* __restrict__ sc,
                                                    double* __restrict__ sc_next) {
   const int c = blockIdx.y;
-  const size_t p = (size_t)blockIdx.x * LBS + threadIdx.x;
   const double* s = sc + (size_t)c * LW_NS;
   const double qq = s[3];
   const double ar = qq > 0.0 ? s[1] / qq : 0.0, ai = qq > 0.0 ? s[2] / qq : 0.0;
   double v[1] = {0.0};
-  if (p < N3) {
+  const R a = (R)ar, b = (R)ai;
+  for (size_t p = (size_t)blockIdx.x * LBS + threadIdx.x; p < N3; p += (size_t)gridDim.x * LBS) {
     const size_t o = (size_t)c * ld + p;
     const Sp<R> z = Z[o], q = Q[o];
     Sp<R> w = W[o], r = Rr[o];
-    const R a = (R)ar, b = (R)ai;
     w.ur -= a * z.ur - b * z.ui;
     w.ui -= a * z.ui + b * z.ur;
     w.dr -= a * z.dr - b * z.di;
@@ -1001,7 +998,7 @@ __global__ __launch_bounds__(LBS) void k_lw_update(const Sp<R>* __restrict__ Z, 
     r.di -= a * q.di + b * q.dr;
     W[o] = w;
     Rr[o] = r;
-    v[0] = (double)r.ur * r.ur + (double)r.ui * r.ui + (double)r.dr * r.dr + (double)r.di * r.di;
+    v[0] += (double)r.ur * r.ur + (double)r.ui * r.ui + (double)r.dr * r.dr + (double)r.di * r.di;
   }
   block_sum_add<1>(v, sc_next + (size_t)c * LW_NS);
 }
@@ -1022,19 +1019,22 @@ __global__ __launch_bounds__(LBS) void k_phase(float* __restrict__ in8, float* _
   }
 }
 
+// the learned step's vector kernels: a capped grid, several points per thread
+// (fewer block reductions and atomics)
+inline dim3 lw_grid(size_t N3, int m) { return dim3((unsigned)std::min<size_t>((N3 + LBS - 1) / LBS, 64), (unsigned)m); }
 template <typename R>
 void launch_lw_init(hipStream_t st, const Sp<R>* X, const Sp<R>* T, const double* ls, Sp<R>* W, Sp<R>* Rr, size_t N3,
                     size_t ld, int m, double* sc) {
-  hipLaunchKernelGGL((k_lw_init<R>), pgrid(N3, m), dim3(LBS), 0, st, X, T, ls, W, Rr, N3, ld, sc);
+  hipLaunchKernelGGL((k_lw_init<R>), lw_grid(N3, m), dim3(LBS), 0, st, X, T, ls, W, Rr, N3, ld, sc);
 }
 template <typename R>
 void launch_lw_dots(hipStream_t st, const Sp<R>* Q, const Sp<R>* Rr, size_t N3, size_t ld, int m, double* sc) {
-  hipLaunchKernelGGL((k_lw_dots<R>), pgrid(N3, m), dim3(LBS), 0, st, Q, Rr, N3, ld, sc);
+  hipLaunchKernelGGL((k_lw_dots<R>), lw_grid(N3, m), dim3(LBS), 0, st, Q, Rr, N3, ld, sc);
 }
 template <typename R>
 void launch_lw_update(hipStream_t st, const Sp<R>* Z, const Sp<R>* Q, Sp<R>* W, Sp<R>* Rr, size_t N3, size_t ld, int m,
                       const double* sc, double* sc_next) {
-  hipLaunchKernelGGL((k_lw_update<R>), pgrid(N3, m), dim3(LBS), 0, st, Z, Q, W, Rr, N3, ld, sc, sc_next);
+  hipLaunchKernelGGL((k_lw_update<R>), lw_grid(N3, m), dim3(LBS), 0, st, Z, Q, W, Rr, N3, ld, sc, sc_next);
 }
 #define LW_INST(R)                                                                                              \
   template void launch_lw_init<R>(hipStream_t, const Sp<R>*, const Sp<R>*, const double*, Sp<R>*, Sp<R>*, size_t, \
