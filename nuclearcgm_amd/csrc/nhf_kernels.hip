@@ -2381,19 +2381,65 @@ __global__ void k_dst_scale(double* __restrict__ x, int n, double h) {
   const double si = sin(c * (i + 1)), sj = sin(c * (j + 1)), sk = sin(c * (k + 1));
   x[p] /= (4.0 / (h * h)) * (si * si + sj * sj + sk * sk);
 }
-// z = M^-1 r (tmp: scratch of n^3)
+// the transform along z, the spectral scaling and the inverse transform
+// along z, fused: a workgroup takes W = BS / n lines (x, y) (thread (line,
+// index along z)), keeps them in LDS between the two transforms (S is its
+// own inverse); needs n <= BS
+__global__ __launch_bounds__(BS) void k_dst_zline(const double* __restrict__ in, double* __restrict__ out,
+                                                  const double* __restrict__ S, int n, double h) {
+  extern __shared__ double sm[];  // S (n x n), then two W x n line images
+  const int W = BS / n;
+  double* Ss = sm;
+  double* a = sm + n * n;
+  double* b = a + W * n;
+  for (int e = threadIdx.x; e < n * n; e += BS) Ss[e] = S[e];
+  const int tx = threadIdx.x % W, tz = threadIdx.x / W;
+  const int line = blockIdx.x * W + tx;  // x + n y
+  const bool act = tz < n && line < n * n;
+  const size_t n2 = (size_t)n * n;
+  if (act) a[tx * n + tz] = in[(size_t)line + n2 * tz];
+  __syncthreads();
+  if (act) {
+    double s = 0.0;
+    for (int t = 0; t < n; ++t) s += Ss[tz * n + t] * a[tx * n + t];
+    const int i = line % n, j = line / n;
+    const double c = PI / (2.0 * (n + 1));
+    const double si = sin(c * (i + 1)), sj = sin(c * (j + 1)), sk = sin(c * (tz + 1));
+    b[tx * n + tz] = s / ((4.0 / (h * h)) * (si * si + sj * sj + sk * sk));
+  }
+  __syncthreads();
+  if (act) {
+    double s = 0.0;
+    for (int t = 0; t < n; ++t) s += Ss[tz * n + t] * b[tx * n + t];
+    out[(size_t)line + n2 * tz] = s;
+  }
+}
+// z = M^-1 r (tmp: scratch of n^3).  The transforms along different axes
+// commute, so the inverse runs z, y, x and the chain ends in z:
+// r -x-> z -y-> tmp -(z, scale, z)-> z -y-> tmp -x-> z
 void launch_dst_precond(hipStream_t st, const double* r, double* z, double* tmp, const double* S, int n, double h) {
   const size_t N3 = (size_t)n * n * n;
   const dim3 g((unsigned)((N3 + BS - 1) / BS));
   const size_t sh = sizeof(double) * n * n;
+  if (n > BS) {  // lines longer than a workgroup: the unfused chain
+    hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, r, z, S, n, 0);
+    hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, z, tmp, S, n, 1);
+    hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, tmp, z, S, n, 2);
+    hipLaunchKernelGGL(k_dst_scale, g, dim3(BS), 0, st, z, n, h);
+    hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, z, tmp, S, n, 0);
+    hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, tmp, z, S, n, 1);
+    hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, z, tmp, S, n, 2);
+    (void)hipMemcpyAsync(z, tmp, sizeof(double) * N3, hipMemcpyDeviceToDevice, st);
+    return;
+  }
+  const int W = BS / n;
+  const dim3 gz((unsigned)((n * n + W - 1) / W));
+  const size_t shz = sizeof(double) * ((size_t)n * n + 2 * (size_t)W * n);
   hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, r, z, S, n, 0);
   hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, z, tmp, S, n, 1);
-  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, tmp, z, S, n, 2);
-  hipLaunchKernelGGL(k_dst_scale, g, dim3(BS), 0, st, z, n, h);
-  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, z, tmp, S, n, 0);
-  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, tmp, z, S, n, 1);
-  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, z, tmp, S, n, 2);
-  (void)hipMemcpyAsync(z, tmp, sizeof(double) * N3, hipMemcpyDeviceToDevice, st);
+  hipLaunchKernelGGL(k_dst_zline, gz, dim3(BS), shz, st, tmp, z, S, n, h);
+  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, z, tmp, S, n, 1);
+  hipLaunchKernelGGL(k_axis_apply, g, dim3(BS), sh, st, tmp, z, S, n, 0);
 }
 // PCG pieces: out += r . z ; p = z + beta p with beta from scal
 __global__ __launch_bounds__(BS) void k_dot(const double* __restrict__ a, const double* __restrict__ b, size_t N3,
